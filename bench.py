@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the WADGES RK stage on one MI355X.
+
+Workload (default): 3D Taylor-Green vortex on the warped periodic cube,
+32^3 cubes x 6 tetrahedra (K = 196,608), N = 3 (Np = 20, Nq = 23, Nf = 48),
+FP64, tau = 1.  A step is one LSRK45 time step = 5 stages of
+vol -> surf -> update over all elements, with every array resident in HBM.
+
+Metric: GDOF.RK-stage/s = K * Np * Nfields * stages / time / 1e9.
+Prints one JSON line (rank 0).  The roofline object is for the volume kernel
+(the dominant one): achieved = algorithmic FP64 flops per launch (workmodel)
+/ mean launch time measured with HIP events on the launch stream.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=32, help="cubes per direction")
+    ap.add_argument("--N", type=int, default=3, help="polynomial degree")
+    ap.add_argument("--affine", action="store_true")
+    ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
+    ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(prob, target_s):
+    """The CPU oracle (OpenMP over elements) timed on the host cores on a sample
+    of elements of the same workload; returns GDOF.stage/s."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    O.build()
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    st = {k: v.copy() for k, v in prob.copy_state().items()}
+    rf = np.zeros_like(st["Qf"])
+    par = O.params(prob)
+    L = O.lib()
+    import ctypes
+
+    def stage(Ks):
+        p = O._p
+        L.wo_vol(ctypes.byref(par), Ks, p(prob.vgeo), p(prob.vfgeo), p(prob.nrJ), p(prob.nsJ),
+                 p(prob.ntJ), p(prob.Drq), p(prob.Dsq), p(prob.Dtq), p(prob.VqLq), p(prob.VfPq),
+                 p(st["Qf"]), p(st["rhsQ"]), p(rf))
+        m = np.ascontiguousarray(prob.mapPq, dtype=np.uint32)
+        L.wo_surf(ctypes.byref(par), Ks, p(prob.fgeo), m.ctypes.data_as(O._up), p(prob.VqLq),
+                  p(st["Qf"]), p(rf), p(st["rhsQ"]))
+        L.wo_update(ctypes.byref(par), Ks, p(prob.Jq), p(prob.VqPq), p(prob.VfPq), 0.0, 0.1,
+                    1e-4, p(st["rhsQ"]), p(st["resQ"]), p(st["Q"]), p(st["Qf"]))
+
+    Ks = min(prob.K, 256 * threads)
+    t0 = time.perf_counter()
+    stage(Ks)
+    t1 = time.perf_counter() - t0
+    Ks = int(min(prob.K, max(Ks, Ks * target_s / max(t1, 1e-6))))
+    Ks -= Ks % threads if Ks > threads else 0
+    t0 = time.perf_counter()
+    stage(Ks)
+    dt = time.perf_counter() - t0
+    from scratch_wadge_amd.workmodel import dofs
+    gdof = dofs(prob.dim, prob.N, Ks) / dt / 1e9
+    return {"value": gdof, "unit": "GDOF*stage/s", "cores": threads, "kind": "port",
+            "sample": f"one vol+surf+update stage over {Ks} of {prob.K} elements ({dt:.1f} s, "
+                      f"C++ oracle, OpenMP {threads} threads)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the element "
+              f"partition across GPUs is designed but not built)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from scratch_wadge_amd import problem as P
+    from scratch_wadge_amd import workmodel as W
+    from scratch_wadge_amd.device import DeviceSolver
+
+    t0 = time.perf_counter()
+    prob = P.tgv3d_problem(n=args.n, N=args.N, curved=not args.affine)
+    setup_s = time.perf_counter() - t0
+    print(f"# setup K={prob.K} N={prob.N} Nq={prob.Nq} Nf={prob.Nf} in {setup_s:.1f}s",
+          file=sys.stderr, flush=True)
+    dev = DeviceSolver(prob)
+    dt = prob.dt
+    for _ in range(args.warmup):
+        dev.step(dt)
+    torch.cuda.synchronize()
+
+    # timed region: exactly `steps` LSRK45 steps; HIP events bracket every
+    # volume launch on the launch stream
+    stream = torch.cuda.current_stream()
+    ev = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        for s in range(5):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            dev.vol()
+            b.record(stream)
+            dev.surf()
+            c = torch.cuda.Event(enable_timing=True)
+            c.record(stream)
+            dev.update(P.RK4A[s], P.RK4B[s], dt)
+            d = torch.cuda.Event(enable_timing=True)
+            d.record(stream)
+            ev.append((a, b, c, d))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    vol_ms = float(np.mean([a.elapsed_time(b) for a, b, c, d in ev]))
+    surf_ms = float(np.mean([b.elapsed_time(c) for a, b, c, d in ev]))
+    upd_ms = float(np.mean([c.elapsed_time(d) for a, b, c, d in ev]))
+
+    stages = 5 * args.steps
+    value = W.dofs(prob.dim, prob.N, prob.K) * stages / elapsed / 1e9
+    flops = W.vol_flops(prob.dim, prob.Nq, prob.Nf) * prob.K
+    achieved = flops / (vol_ms * 1e-3) / 1e12
+    st = dev.get_state()
+    if not all(np.isfinite(v).all() for v in st.values()):
+        print("# state is not finite after the run", file=sys.stderr)
+        sys.exit(1)
+
+    out = {
+        "metric": "GDOF*RK-stage/s (3D curved tet N=3, FP64)",
+        "value": round(value, 4),
+        "unit": "GDOF*stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green initial state on a generated warped periodic tet mesh)",
+        "config": {"workload": f"3D TGV {'affine' if args.affine else 'curved (warp 0.1)'} tets, "
+                               f"{args.n}^3 x 6 = {prob.K} elements, N={prob.N}",
+                   "K": prob.K, "N": prob.N, "Np": prob.Np, "Nq": prob.Nq, "Nf": prob.Nf,
+                   "stages_per_step": 5, "tau": prob.tau},
+        "kernels_ms": {"vol": round(vol_ms, 4), "surf": round(surf_ms, 4), "update": round(upd_ms, 4)},
+        "roofline": {
+            "bound": "mfma", "pipe": "fp64 vector ALU (its published peak equals the FP64 matrix peak)",
+            "kernel": "vol_kernel",
+            "achieved": round(achieved, 3), "peak": W.FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / W.FP64_PEAK_TFLOPS, 4),
+            "flops_per_launch": flops, "traffic": None,
+        },
+        "hbm": {
+            "surf_GBs": round(W.bytes_surf(prob.dim, prob.Nq, prob.Nf) * prob.K / (surf_ms * 1e-3) / 1e9, 1),
+            "update_GBs": round(W.bytes_update(prob.dim, prob.Nq, prob.Nf) * prob.K / (upd_ms * 1e-3) / 1e9, 1),
+            "vol_GBs": round(W.bytes_vol(prob.dim, prob.Nq, prob.Nf) * prob.K / (vol_ms * 1e-3) / 1e9, 1),
+        },
+        "cpu_baseline": None,
+    }
+    if args.cpu:
+        out["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

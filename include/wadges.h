@@ -1,0 +1,101 @@
+/*
+ * wadges.h -- C ABI of the MI355X (gfx950) WADGES right-hand-side / time-step
+ * hot path: hand-written HIP kernels for the volume flux-differencing, surface
+ * entropy-stable flux and weight-adjusted inverse-mass + LSRK update kernels.
+ *
+ * Drop-in boundary.  The reference driver launches three OKL kernels per RK
+ * stage through OCCA (euler.c:3610-3670):
+ *
+ *   vol    (K, vgeo, vfgeo, nrJ, nsJ, [ntJ], Drq, Dsq, [Dtq, Drstq], VqLq, VfPq,
+ *           Q, Qf, rhsQ, rhsQf)                     euler.c:3615-3617 / :3656-3658
+ *   surf   (K, fgeo, mapPq, VqLq, Qf, rhsQf, rhsQ)  euler.c:3619-3620 / :3660-3661
+ *   update (K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ, Q, Qf)
+ *                                                   euler.c:3622-3625 / :3663-3666
+ *
+ * wadges_vol / wadges_surf / wadges_update keep those roles, argument order and
+ * array layouts; the OCCA compile-time defines (p_Nq, p_Nfq, p_Nfaces,
+ * p_Nfields, p_gamma, p_TAU, p_Kblk*; euler.c:2880-2948) become a plan.  Every
+ * pointer is a device pointer, every call is asynchronous on the given HIP
+ * stream (a hipStream_t passed as void*, NULL = default stream), no call
+ * allocates or synchronises, and the calls can be captured in a HIP graph.
+ *
+ * Layouts (Appendix of the reference's operators_c.cpp:322-389 and
+ * euler.c:3158-3170), all FP64 except mapPq:
+ *   Q, rhsQ, resQ [K][Nfields][Nq]     Qf, rhsQf [K][Nfields][Nf], Nf = Nfq*Nfaces
+ *   vgeo [K][Nvgeo][Nq]  vfgeo [K][Nvgeo][Nf]  fgeo [K][Nfgeo][Nf]  Jq [K][Nq]
+ *   mapPq [K][Nf] (uint32): e_nbr*Nf*Nfields + neighbour face point
+ *   Drq, Dsq, Dtq, VqPq [Nq x Nq], VqLq [Nq x Nf], VfPq [Nf x Nq], column major
+ *   nrJ, nsJ, ntJ [Nf]
+ * with Nfields = dim+2, Nvgeo = dim*dim, Nfgeo = dim+1, Nfaces = dim+1.
+ *
+ * Return codes: 0 on success, negative on error (see WADGES_E*); the library
+ * never aborts the process.
+ */
+#ifndef WADGES_H
+#define WADGES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  WADGES_OK = 0,
+  WADGES_EINVAL = -1,       /* bad argument (null pointer, K < 0, size mismatch) */
+  WADGES_EUNSUPPORTED = -2, /* (dim, Nq, Nfq) has no compiled kernel instance   */
+  WADGES_EHIP = -3,         /* a HIP runtime call failed (wadges_last_hip_error) */
+  WADGES_ENOMEM = -4        /* device allocation at plan creation failed         */
+};
+
+typedef struct wadges_plan wadges_plan;
+
+/* Creates a plan: the reference's kernel defines (euler.c:2880-2948) as a
+ * runtime object.  dim = 2 (triangles) or 3 (tetrahedra); N the polynomial
+ * degree; Nq, Nfq the volume / per-face quadrature sizes the operators were
+ * built with; gamma = p_gamma; tau = p_TAU (the reference's 2D surface kernel
+ * has no tau and equals tau = 1).  Allocates the plan's small device workspace
+ * on the current device; this is the only call that allocates. */
+int wadges_plan_create(wadges_plan **plan, int dim, int N, int Nq, int Nfq, double gamma,
+                       double tau);
+int wadges_plan_destroy(wadges_plan *plan);
+
+/* Volume kernel: role of euler_vol_3d_curved (okl/Euler3D.okl:2123-2523) and
+ * euler_vol_2d_curved (okl/Euler2D.okl:773-1353).  Reads the entropy-projected
+ * volume state from rhsQ and the face state from Qf; overwrites rhsQ with the
+ * flux-differencing volume term and writes the volume-to-face coupling term to
+ * rhsQf.  Q and Drstq are accepted for argument compatibility and not read (the
+ * reference kernel does not read them either).  2D: ntJ, Dtq, Drstq may be NULL. */
+int wadges_vol(const wadges_plan *plan, int K, const double *vgeo, const double *vfgeo,
+               const double *nrJ, const double *nsJ, const double *ntJ, const double *Drq,
+               const double *Dsq, const double *Dtq, const double *Drstq, const double *VqLq,
+               const double *VfPq, const double *Q, const double *Qf, double *rhsQ,
+               double *rhsQf, void *stream);
+
+/* Surface kernel: role of euler_surf_3d (okl/Euler3D.okl:3242-3473) and
+ * euler_surf_2d (okl/Euler2D.okl:1357-1495).  Entropy-conservative flux plus
+ * Lax-Friedrichs penalty at every face point, plus rhsQf, lifted by VqLq and
+ * added to rhsQ. */
+int wadges_surf(const wadges_plan *plan, int K, const double *fgeo, const uint32_t *mapPq,
+                const double *VqLq, const double *Qf, const double *rhsQf, double *rhsQ,
+                void *stream);
+
+/* Update kernel: role of euler_update_3d_curved (okl/Euler3D.okl:2527-2884) and
+ * euler_update_2d_curved (okl/Euler2D.okl:1729-2282).  Weight-adjusted inverse
+ * mass -2 VqPq diag(1/J) VqPq, low-storage RK update of resQ and Q, then the
+ * weight-adjusted entropy projection: Qf = u(VfPq v~), rhsQ = u(VqPq v~). */
+int wadges_update(const wadges_plan *plan, int K, const double *Jq, const double *VqPq,
+                  const double *VfPq, double fa, double fb, double fdt, double *rhsQ,
+                  double *resQ, double *Q, double *Qf, void *stream);
+
+/* Introspection. */
+int wadges_last_hip_error(void);           /* hipError_t of the last failing HIP call */
+const char *wadges_error_string(int code);  /* text for a WADGES_E* code */
+int wadges_instance_count(void);            /* number of compiled (dim, Nq, Nfq) instances */
+int wadges_instance(int idx, int *dim, int *N, int *Nq, int *Nfq);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* WADGES_H */

@@ -1,0 +1,99 @@
+"""ctypes binding of the C ABI declared in include/wadges.h.
+
+The product path: every kernel call goes through libwadges.so (HIP, gfx950).
+If the library is missing, loading fails with an error -- there is no CPU
+fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwadges.so")
+
+# Every symbol the header declares (checked by the CPU tests).
+EXPORTED = (
+    "wadges_plan_create", "wadges_plan_destroy", "wadges_vol", "wadges_surf", "wadges_update",
+    "wadges_last_hip_error", "wadges_error_string", "wadges_instance_count", "wadges_instance",
+)
+
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class WadgesError(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise WadgesError(f"{LIB_PATH} is missing: build it with `python -m scratch_wadge_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.wadges_plan_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.wadges_plan_destroy.argtypes = [_vp]
+    L.wadges_vol.argtypes = [_vp, ctypes.c_int] + [_vp] * 15 + [_vp]
+    L.wadges_surf.argtypes = [_vp, ctypes.c_int] + [_vp] * 6 + [_vp]
+    L.wadges_update.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, _vp, _vp, _vp, _vp, _vp]
+    L.wadges_error_string.restype = ctypes.c_char_p
+    L.wadges_error_string.argtypes = [ctypes.c_int]
+    L.wadges_instance.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != 0:
+        L = load()
+        msg = L.wadges_error_string(rc).decode()
+        raise WadgesError(f"{what} failed: {msg} (code {rc}, hip error {L.wadges_last_hip_error()})")
+
+
+def instances():
+    L = load()
+    out = []
+    for n in range(L.wadges_instance_count()):
+        v = [ctypes.c_int() for _ in range(4)]
+        check(L.wadges_instance(n, *[ctypes.byref(x) for x in v]), "wadges_instance")
+        out.append(tuple(x.value for x in v))
+    return out
+
+
+class Plan:
+    """Owns a wadges_plan (the reference's kernel defines as a runtime object)."""
+
+    def __init__(self, dim, N, Nq, Nfq, gamma, tau):
+        L = load()
+        h = _vp()
+        check(L.wadges_plan_create(ctypes.byref(h), dim, N, Nq, Nfq, gamma, tau), "wadges_plan_create")
+        self.h = h
+        self.dim, self.N, self.Nq, self.Nfq = dim, N, Nq, Nfq
+
+    def close(self):
+        if self.h:
+            load().wadges_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def vol(self, K, vgeo, vfgeo, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, Drstq, VqLq, VfPq, Q, Qf, rhsQ,
+            rhsQf, stream=None):
+        check(load().wadges_vol(self.h, K, vgeo, vfgeo, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, Drstq, VqLq,
+                                VfPq, Q, Qf, rhsQ, rhsQf, stream), "wadges_vol")
+
+    def surf(self, K, fgeo, mapPq, VqLq, Qf, rhsQf, rhsQ, stream=None):
+        check(load().wadges_surf(self.h, K, fgeo, mapPq, VqLq, Qf, rhsQf, rhsQ, stream), "wadges_surf")
+
+    def update(self, K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ, Q, Qf, stream=None):
+        check(load().wadges_update(self.h, K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ, Q, Qf, stream),
+              "wadges_update")

@@ -1,0 +1,659 @@
+// wadges.hip -- MI355X (gfx950) kernels and C ABI of the WADGES hot path.
+//
+// Three kernels per LSRK stage, with the roles and argument layouts of the
+// reference's OKL kernels (see include/wadges.h for the file:line map):
+//
+//   vol_kernel     flux-differencing volume term and volume<->face coupling.
+//                  FP64 VALU bound.  One team of Nq lanes per element, teams
+//                  packed inside one 64-wide wave.  Lane i owns volume row i
+//                  in registers; partner rows (volume row j of a symmetric
+//                  volume pair, face row f of a volume-face pair) are
+//                  accumulated in LDS.  Pairs are visited on a diagonal
+//                  schedule (partner = (i + t) mod L) so that at any step the
+//                  lanes of a team touch distinct partner rows: each two-point
+//                  flux is evaluated once (Nq(Nq+1)/2 + Nq*Nf per element,
+//                  versus Nq^2 + 2 Nq Nf in the reference kernel) and the LDS
+//                  accumulation order is fixed, so results are deterministic.
+//   surf_kernel    entropy-conservative flux + Lax-Friedrichs penalty at the
+//                  face points, gathered through mapPq, lifted by VqLq.
+//                  HBM bound.
+//   update_kernel  weight-adjusted inverse mass, LSRK update, weight-adjusted
+//                  entropy projection and u(v) at volume and face points.
+//                  HBM / transcendental bound; operators staged in LDS.
+//
+// Operator tables of vol_kernel are re-laid out per call by a tiny prep kernel
+// into the plan's workspace, "diagonal-major": entry [t][i] belongs to the pair
+// (i, (i + t) mod L), so at step t the lanes of every team read consecutive
+// addresses that all teams of all waves share (L1 hits).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "wadges_device.hpp"
+#include "../../include/wadges.h"
+
+namespace wadges {
+
+constexpr int WAVE = 64;
+
+template <int A, int B> struct Max { static constexpr int value = A > B ? A : B; };
+
+// Compile-time shape of one (dim, Nq, Nfq) instance.
+template <int DIM_, int NQ_, int NFQ_> struct Shape {
+  static constexpr int DIM = DIM_;
+  static constexpr int NQ = NQ_;
+  static constexpr int NFQ = NFQ_;
+  static constexpr int NFACES = DIM + 1;
+  static constexpr int NFN = NFQ * NFACES;   // face points per element (Nf)
+  static constexpr int NFLD = DIM + 2;
+  static constexpr int NG = DIM * DIM;
+  static constexpr int L = Max<NQ, NFN>::value;  // diagonal period of the vol-face block
+  static constexpr int TVT = NQ / 2 + 1;        // diagonal steps of the vol-vol block
+  // node record layouts (structure of arrays in LDS)
+  static constexpr int VS = DIM + 5;            // rho, hv[DIM], beta, lrho, lbeta, hq
+  static constexpr int VREC = VS + NG;          // + half geofacs
+  static constexpr int FREC = VS + DIM;         // + b_d = sum_r hGf_rd nref_r
+  // workspace layout (doubles)
+  static constexpr int WS_D = 2 * DIM * TVT * NQ;   // D fwd / bwd
+  static constexpr int WS_W = 2 * L * NQ;           // +1/2 VqLq, -1/2 VfPq
+  static constexpr int WS = WS_D + WS_W;
+};
+
+// ------------------------------------------------------------------ operator prep
+
+template <class S>
+__global__ void prep_vol_tables(const double *__restrict__ Drq, const double *__restrict__ Dsq,
+                                const double *__restrict__ Dtq, const double *__restrict__ VqLq,
+                                const double *__restrict__ VfPq, double *__restrict__ ws) {
+  constexpr int NQ = S::NQ, NFN = S::NFN, L = S::L, TVT = S::TVT, DIM = S::DIM;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < S::WS_D) {
+    const int i = idx % NQ;
+    const int t = (idx / NQ) % TVT;
+    const int r = (idx / (NQ * TVT)) % DIM;
+    const int s = idx / (NQ * TVT * DIM);
+    int j = i + t;
+    if (j >= NQ) j -= NQ;
+    const double *D = r == 0 ? Drq : (r == 1 ? Dsq : Dtq);
+    ws[idx] = s == 0 ? D[i + j * NQ] : D[j + i * NQ];
+  } else if (idx < S::WS) {
+    const int k = idx - S::WS_D;
+    const int i = k % NQ;
+    const int t = (k / NQ) % L;
+    const int s = k / (NQ * L);
+    int f = i + t;
+    if (f >= L) f -= L;
+    double v = 0.0;
+    if (f < NFN) v = s == 0 ? 0.5 * VqLq[i + f * NQ] : -0.5 * VfPq[f + i * NFN];
+    ws[idx] = v;
+  }
+}
+
+// ------------------------------------------------------------------ volume kernel
+
+template <class S> struct VolSmem {
+  static constexpr int per_elem =
+      S::VREC * S::NQ + S::FREC * S::NFN + S::NFLD * S::NQ + S::NFLD * S::NFN;
+};
+
+template <class S, int TPW>
+__global__ void __launch_bounds__(WAVE)
+vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vfgeo,
+           const double *__restrict__ nrJ, const double *__restrict__ nsJ,
+           const double *__restrict__ ntJ, const double *__restrict__ ws,
+           const double *__restrict__ Qf, double *rhsQ, double *__restrict__ rhsQf,
+           double gm1, double inv2gm1) {
+  constexpr int DIM = S::DIM, NQ = S::NQ, NFQ = S::NFQ, NFN = S::NFN, NFLD = S::NFLD;
+  constexpr int NG = S::NG, L = S::L, TVT = S::TVT, VS = S::VS, VREC = S::VREC, FREC = S::FREC;
+  constexpr int PER = VolSmem<S>::per_elem;
+  extern __shared__ double smem[];
+
+  const int lane = threadIdx.x;
+  const int team = lane / NQ;
+  const int i = lane - team * NQ;
+  const bool active = team < TPW;
+  const int k = blockIdx.x * TPW + team;
+  const bool valid = active && k < K;
+
+  __shared__ double s_nref[S::NFACES][DIM];
+  if (lane < S::NFACES) {
+    s_nref[lane][0] = nrJ[lane * NFQ];
+    s_nref[lane][1] = nsJ[lane * NFQ];
+    if constexpr (DIM == 3) s_nref[lane][2] = ntJ[lane * NFQ];
+  }
+  __syncthreads();
+
+  double *vrec = smem + (active ? team : 0) * PER;
+  double *frec = vrec + VREC * NQ;
+  double *accV = frec + FREC * NFN;
+  double *accF = accV + NFLD * NQ;
+
+  Node<DIM> me;
+  double hG[NG];
+  if (valid) {
+    // own volume node
+    double U[NFLD];
+    const double *src = rhsQ + (size_t)k * NQ * NFLD;
+    for (int f = 0; f < NFLD; ++f) U[f] = src[i + f * NQ];
+    me = make_node<DIM>(U, gm1);
+    const double *g = vgeo + (size_t)k * NQ * NG;
+    for (int c = 0; c < NG; ++c) hG[c] = 0.5 * g[i + c * NQ];
+    vrec[0 * NQ + i] = me.rho;
+    for (int d = 0; d < DIM; ++d) vrec[(1 + d) * NQ + i] = me.hv[d];
+    vrec[(DIM + 1) * NQ + i] = me.beta;
+    vrec[(DIM + 2) * NQ + i] = me.lrho;
+    vrec[(DIM + 3) * NQ + i] = me.lbeta;
+    vrec[(DIM + 4) * NQ + i] = me.hq;
+    for (int c = 0; c < NG; ++c) vrec[(VS + c) * NQ + i] = hG[c];
+    for (int f = 0; f < NFLD; ++f) accV[f * NQ + i] = 0.0;
+    // face nodes owned by this lane
+    const double *srcf = Qf + (size_t)k * NFN * NFLD;
+    const double *gf = vfgeo + (size_t)k * NFN * NG;
+    for (int fn = i; fn < NFN; fn += NQ) {
+      double Uf[NFLD];
+      for (int f = 0; f < NFLD; ++f) Uf[f] = srcf[fn + f * NFN];
+      const Node<DIM> nf = make_node<DIM>(Uf, gm1);
+      frec[0 * NFN + fn] = nf.rho;
+      for (int d = 0; d < DIM; ++d) frec[(1 + d) * NFN + fn] = nf.hv[d];
+      frec[(DIM + 1) * NFN + fn] = nf.beta;
+      frec[(DIM + 2) * NFN + fn] = nf.lrho;
+      frec[(DIM + 3) * NFN + fn] = nf.lbeta;
+      frec[(DIM + 4) * NFN + fn] = nf.hq;
+      const int face = fn / NFQ;
+      for (int d = 0; d < DIM; ++d) {
+        double b = 0.0;
+        for (int r = 0; r < DIM; ++r) b = b + 0.5 * gf[fn + (r * DIM + d) * NFN] * s_nref[face][r];
+        frec[(VS + d) * NFN + fn] = b;
+      }
+      for (int f = 0; f < NFLD; ++f) accF[f * NFN + fn] = 0.0;
+    }
+  }
+  __syncthreads();
+
+  double acc[NFLD];
+  for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
+
+  if (valid) {
+    // ---------------- volume-volume block: pairs {i, (i+t) mod NQ}, t = 0..NQ/2
+    const double *Dfw = ws;
+    const double *Dbw = ws + DIM * TVT * NQ;
+    for (int t = 0; t < TVT; ++t) {
+      if (2 * t > NQ) break;
+      if (2 * t == NQ && i >= NQ / 2) continue;  // even NQ: middle pair once
+      int j = i + t;
+      if (j >= NQ) j -= NQ;
+      Node<DIM> pj;
+      pj.rho = vrec[0 * NQ + j];
+      for (int d = 0; d < DIM; ++d) pj.hv[d] = vrec[(1 + d) * NQ + j];
+      pj.beta = vrec[(DIM + 1) * NQ + j];
+      pj.lrho = vrec[(DIM + 2) * NQ + j];
+      pj.lbeta = vrec[(DIM + 3) * NQ + j];
+      pj.hq = vrec[(DIM + 4) * NQ + j];
+      double Ga[NG];
+      for (int c = 0; c < NG; ++c) Ga[c] = hG[c] + vrec[(VS + c) * NQ + j];
+      const PairFlux<DIM> P = pair_flux<DIM>(me, pj, inv2gm1);
+      double Dx[DIM];
+      for (int d = 0; d < DIM; ++d) Dx[d] = 0.0;
+      for (int r = 0; r < DIM; ++r) {
+        const double Dr = Dfw[(r * TVT + t) * NQ + i];
+        for (int d = 0; d < DIM; ++d) Dx[d] = Dx[d] + Dr * Ga[r * DIM + d];
+      }
+      double c[NFLD];
+      flux_dot<DIM>(P, Dx, c);
+      for (int f = 0; f < NFLD; ++f) acc[f] += c[f];
+      if (t > 0) {
+        for (int d = 0; d < DIM; ++d) Dx[d] = 0.0;
+        for (int r = 0; r < DIM; ++r) {
+          const double Dr = Dbw[(r * TVT + t) * NQ + i];
+          for (int d = 0; d < DIM; ++d) Dx[d] = Dx[d] + Dr * Ga[r * DIM + d];
+        }
+        flux_dot<DIM>(P, Dx, c);
+        for (int f = 0; f < NFLD; ++f) lds_add(&accV[f * NQ + j], c[f]);
+      }
+    }
+
+    // ---------------- volume-face block: pairs (i, (i+t) mod L), t = 0..L-1
+    const double *W1 = ws + S::WS_D;
+    const double *W2 = W1 + L * NQ;
+    for (int t = 0; t < L; ++t) {
+      int fn = i + t;
+      if (fn >= L) fn -= L;
+      if (fn >= NFN) continue;
+      Node<DIM> pf;
+      pf.rho = frec[0 * NFN + fn];
+      for (int d = 0; d < DIM; ++d) pf.hv[d] = frec[(1 + d) * NFN + fn];
+      pf.beta = frec[(DIM + 1) * NFN + fn];
+      pf.lrho = frec[(DIM + 2) * NFN + fn];
+      pf.lbeta = frec[(DIM + 3) * NFN + fn];
+      pf.hq = frec[(DIM + 4) * NFN + fn];
+      const int face = fn / NFQ;
+      double n[DIM];
+      for (int d = 0; d < DIM; ++d) {
+        double a = frec[(VS + d) * NFN + fn];
+        for (int r = 0; r < DIM; ++r) a = a + hG[r * DIM + d] * s_nref[face][r];
+        n[d] = a;
+      }
+      const PairFlux<DIM> P = pair_flux<DIM>(me, pf, inv2gm1);
+      double Fn[NFLD];
+      flux_dot<DIM>(P, n, Fn);
+      const double w1 = W1[t * NQ + i];
+      const double w2 = W2[t * NQ + i];
+      for (int f = 0; f < NFLD; ++f) {
+        acc[f] += w1 * Fn[f];
+        lds_add(&accF[f * NFN + fn], w2 * Fn[f]);
+      }
+    }
+  }
+  __syncthreads();
+
+  if (valid) {
+    double *dst = rhsQ + (size_t)k * NQ * NFLD;
+    for (int f = 0; f < NFLD; ++f) dst[i + f * NQ] = acc[f] + accV[f * NQ + i];
+    double *dstf = rhsQf + (size_t)k * NFN * NFLD;
+    for (int fn = i; fn < NFN; fn += NQ)
+      for (int f = 0; f < NFLD; ++f) dstf[fn + f * NFN] = accF[f * NFN + fn];
+  }
+}
+
+// ------------------------------------------------------------------ surface kernel
+
+template <class S, int TS, int TPW>
+__global__ void __launch_bounds__(WAVE * 4)
+surf_kernel(int K, const double *__restrict__ fgeo, const uint32_t *__restrict__ mapPq,
+            const double *__restrict__ VqLq, const double *__restrict__ Qf,
+            const double *__restrict__ rhsQf, double *__restrict__ rhsQ, double gamma,
+            double gm1, double inv2gm1, double ltau) {
+  constexpr int DIM = S::DIM, NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD, NFG = DIM + 1;
+  constexpr int EPB = 4 * TPW;
+  __shared__ double sfS[EPB][NFLD][NFN];
+  const int wave = threadIdx.x / WAVE;
+  const int wl = threadIdx.x % WAVE;
+  const int tw = wl / TS;
+  const int lane = wl - tw * TS;
+  const bool active = tw < TPW;
+  const int e = wave * TPW + (active ? tw : 0);
+  const int k = blockIdx.x * EPB + e;
+  const bool valid = active && k < K;
+
+  if (valid && lane < NFN) {
+    const int fn = lane;
+    const size_t idM = (size_t)fn + (size_t)k * NFN * NFLD;
+    const size_t idP = mapPq[(size_t)k * NFN + fn];
+    double UM[NFLD], UP[NFLD];
+    for (int f = 0; f < NFLD; ++f) {
+      UM[f] = Qf[idM + (size_t)f * NFN];
+      UP[f] = Qf[idP + (size_t)f * NFN];
+    }
+    const double *g = fgeo + (size_t)k * NFN * NFG;
+    double nJ[DIM];
+    for (int d = 0; d < DIM; ++d) nJ[d] = g[fn + d * NFN];
+    const double sJ = g[fn + DIM * NFN];
+    const Node<DIM> a = make_node<DIM>(UM, gm1);
+    const Node<DIM> b = make_node<DIM>(UP, gm1);
+    // wave speed |u| + sqrt(gamma p / rho)  (okl/Euler3D.okl:3316-3321); 2 hq = |u|^2 exactly
+    const double pM = gm1 * (UM[NFLD - 1] - 0.5 * UM[0] * (2.0 * a.hq));
+    const double pP = gm1 * (UP[NFLD - 1] - 0.5 * UP[0] * (2.0 * b.hq));
+    const double lamM = sqrt(2.0 * a.hq) + sqrt(gamma * pM / UM[0]);
+    const double lamP = sqrt(2.0 * b.hq) + sqrt(gamma * pP / UP[0]);
+    const double Lfc = ltau * fmax(lamM, lamP) * sJ;
+    const PairFlux<DIM> P = pair_flux<DIM>(a, b, inv2gm1);
+    double Fn[NFLD];
+    flux_dot<DIM>(P, nJ, Fn);
+    const double *rf = rhsQf + (size_t)k * NFN * NFLD;
+    for (int f = 0; f < NFLD; ++f)
+      sfS[e][f][fn] = 0.5 * Fn[f] + rf[fn + f * NFN] - Lfc * (UP[f] - UM[f]);
+  }
+  __syncthreads();
+  if (valid) {
+    for (int row = lane; row < NQ; row += TS) {
+      double val[NFLD];
+      for (int f = 0; f < NFLD; ++f) val[f] = 0.0;
+#pragma unroll 2
+      for (int j = 0; j < NFN; ++j) {
+        const double Lij = VqLq[row + j * NQ];
+        for (int f = 0; f < NFLD; ++f) val[f] += Lij * sfS[e][f][j];
+      }
+      double *dst = rhsQ + (size_t)k * NQ * NFLD;
+      for (int f = 0; f < NFLD; ++f) dst[row + f * NQ] += val[f];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ update kernel
+
+template <class S, int TPW>
+__global__ void __launch_bounds__(WAVE * 4)
+update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ VqPq,
+              const double *__restrict__ VfPq, double fa, double fb, double fdt,
+              double *__restrict__ rhsQ, double *__restrict__ resQ, double *__restrict__ Q,
+              double *__restrict__ Qf, double gamma, double lgm1) {
+  constexpr int DIM = S::DIM, NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD;
+  constexpr int EPB = 4 * TPW;
+  __shared__ double sP[NQ * NQ];
+  __shared__ double sPf[NFN * NQ];
+  __shared__ double bufA[EPB][NFLD][NQ];
+  __shared__ double bufB[EPB][NFLD][NQ];
+  for (int idx = threadIdx.x; idx < NQ * NQ; idx += blockDim.x) sP[idx] = VqPq[idx];
+  for (int idx = threadIdx.x; idx < NFN * NQ; idx += blockDim.x) sPf[idx] = VfPq[idx];
+
+  const int wave = threadIdx.x / WAVE;
+  const int wl = threadIdx.x % WAVE;
+  const int tw = wl / NQ;
+  const int i = wl - tw * NQ;
+  const bool active = tw < TPW;
+  const int e = wave * TPW + (active ? tw : 0);
+  const int k = blockIdx.x * EPB + e;
+  const bool valid = active && k < K;
+  const size_t base = (size_t)k * NQ * NFLD;
+
+  double J = 1.0, invJ = 1.0;
+  if (valid) {
+    J = Jq[(size_t)k * NQ + i];
+    invJ = 1.0 / J;
+    for (int f = 0; f < NFLD; ++f) bufA[e][f][i] = rhsQ[base + i + f * NQ];
+  }
+  __syncthreads();
+  // r = -2 VqPq rhs ; bufB = r / J
+  if (valid) {
+    double r[NFLD];
+    for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
+#pragma unroll 2
+    for (int j = 0; j < NQ; ++j) {
+      const double P = -2.0 * sP[i + j * NQ];
+      for (int f = 0; f < NFLD; ++f) r[f] += P * bufA[e][f][j];
+    }
+    for (int f = 0; f < NFLD; ++f) bufB[e][f][i] = (DIM == 3) ? r[f] * invJ : r[f] / J;
+  }
+  __syncthreads();
+  // r = VqPq bufB ; RK update ; bufA = v(Q) J
+  if (valid) {
+    double r[NFLD];
+    for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
+#pragma unroll 2
+    for (int j = 0; j < NQ; ++j) {
+      const double P = sP[i + j * NQ];
+      for (int f = 0; f < NFLD; ++f) r[f] += P * bufB[e][f][j];
+    }
+    double U[NFLD], V[NFLD];
+    for (int f = 0; f < NFLD; ++f) {
+      const size_t id = base + i + f * NQ;
+      const double res = fa * resQ[id] + fdt * r[f];
+      resQ[id] = res;
+      U[f] = Q[id] + fb * res;
+      Q[id] = U[f];
+    }
+    entropy_vars<DIM>(U, gamma, V);
+    for (int f = 0; f < NFLD; ++f) bufA[e][f][i] = V[f] * J;
+  }
+  __syncthreads();
+  // bufB = VqPq bufA / J
+  if (valid) {
+    double r[NFLD];
+    for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
+#pragma unroll 2
+    for (int j = 0; j < NQ; ++j) {
+      const double P = sP[i + j * NQ];
+      for (int f = 0; f < NFLD; ++f) r[f] += P * bufA[e][f][j];
+    }
+    for (int f = 0; f < NFLD; ++f) bufB[e][f][i] = (DIM == 3) ? r[f] * invJ : r[f] / J;
+  }
+  __syncthreads();
+  // bufA = VqPq bufB   (entropy variables v~)
+  if (valid) {
+    double r[NFLD];
+    for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
+#pragma unroll 2
+    for (int j = 0; j < NQ; ++j) {
+      const double P = sP[i + j * NQ];
+      for (int f = 0; f < NFLD; ++f) r[f] += P * bufB[e][f][j];
+    }
+    for (int f = 0; f < NFLD; ++f) bufA[e][f][i] = r[f];
+  }
+  __syncthreads();
+  if (valid) {
+    // face points: Qf = u(VfPq v~)
+    const size_t basef = (size_t)k * NFN * NFLD;
+    for (int fn = i; fn < NFN; fn += NQ) {
+      double V[NFLD], U[NFLD];
+      for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
+#pragma unroll 2
+      for (int j = 0; j < NQ; ++j) {
+        const double P = sPf[fn + j * NFN];
+        for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[e][f][j];
+      }
+      conservative_vars<DIM>(V, gamma, lgm1, U);
+      for (int f = 0; f < NFLD; ++f) Qf[basef + fn + f * NFN] = U[f];
+    }
+    // volume points: rhsQ = u(VqPq v~)
+    double V[NFLD], U[NFLD];
+    for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
+#pragma unroll 2
+    for (int j = 0; j < NQ; ++j) {
+      const double P = sP[i + j * NQ];
+      for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[e][f][j];
+    }
+    conservative_vars<DIM>(V, gamma, lgm1, U);
+    for (int f = 0; f < NFLD; ++f) rhsQ[base + i + f * NQ] = U[f];
+  }
+}
+
+// ------------------------------------------------------------------ dispatch
+
+thread_local int g_last_hip = 0;
+
+inline int hip_check(hipError_t e) {
+  if (e != hipSuccess) {
+    g_last_hip = (int)e;
+    return WADGES_EHIP;
+  }
+  return WADGES_OK;
+}
+
+struct Ops {
+  int (*vol)(const wadges_plan *, int, const double *, const double *, const double *,
+             const double *, const double *, const double *, const double *, const double *,
+             const double *, const double *, const double *, double *, double *, hipStream_t);
+  int (*surf)(const wadges_plan *, int, const double *, const uint32_t *, const double *,
+              const double *, const double *, double *, hipStream_t);
+  int (*update)(const wadges_plan *, int, const double *, const double *, const double *,
+                double, double, double, double *, double *, double *, double *, hipStream_t);
+  int ws_doubles;
+};
+
+}  // namespace wadges
+
+struct wadges_plan {
+  int dim, N, Nq, Nfq, Nfaces, Nf, nfields;
+  double gamma, tau;
+  double *ws;  // device workspace (operator tables)
+  int device;
+  wadges::Ops ops;
+};
+
+namespace wadges {
+
+template <class S> int vol_impl(const wadges_plan *p, int K, const double *vgeo,
+                                const double *vfgeo, const double *nrJ, const double *nsJ,
+                                const double *ntJ, const double *Drq, const double *Dsq,
+                                const double *Dtq, const double *VqLq, const double *VfPq,
+                                const double *Qf, double *rhsQ, double *rhsQf, hipStream_t st) {
+  if (K == 0) return WADGES_OK;
+  {
+    const int n = S::WS;
+    prep_vol_tables<S><<<(n + 255) / 256, 256, 0, st>>>(Drq, Dsq, Dtq, VqLq, VfPq, p->ws);
+    int rc = hip_check(hipGetLastError());
+    if (rc) return rc;
+  }
+  constexpr int TPW = WAVE / S::NQ;
+  static_assert(TPW >= 1, "Nq must fit in one wave");
+  const size_t smem = (size_t)TPW * VolSmem<S>::per_elem * sizeof(double);
+  const int blocks = (K + TPW - 1) / TPW;
+  vol_kernel<S, TPW><<<blocks, WAVE, smem, st>>>(K, vgeo, vfgeo, nrJ, nsJ, ntJ, p->ws, Qf, rhsQ,
+                                                 rhsQf, p->gamma - 1.0,
+                                                 1.0 / (2.0 * (p->gamma - 1.0)));
+  return hip_check(hipGetLastError());
+}
+
+template <class S> int surf_impl(const wadges_plan *p, int K, const double *fgeo,
+                                 const uint32_t *mapPq, const double *VqLq, const double *Qf,
+                                 const double *rhsQf, double *rhsQ, hipStream_t st) {
+  if (K == 0) return WADGES_OK;
+  constexpr int TS = Max<S::NQ, S::NFN>::value;
+  constexpr int TPW = WAVE / TS;
+  static_assert(TPW >= 1, "face points must fit in one wave");
+  constexpr int EPB = 4 * TPW;
+  const int blocks = (K + EPB - 1) / EPB;
+  surf_kernel<S, TS, TPW><<<blocks, WAVE * 4, 0, st>>>(K, fgeo, mapPq, VqLq, Qf, rhsQf, rhsQ,
+                                                       p->gamma, p->gamma - 1.0,
+                                                       1.0 / (2.0 * (p->gamma - 1.0)),
+                                                       0.25 * p->tau);
+  return hip_check(hipGetLastError());
+}
+
+template <class S> int update_impl(const wadges_plan *p, int K, const double *Jq,
+                                   const double *VqPq, const double *VfPq, double fa, double fb,
+                                   double fdt, double *rhsQ, double *resQ, double *Q, double *Qf,
+                                   hipStream_t st) {
+  if (K == 0) return WADGES_OK;
+  constexpr int TPW = WAVE / S::NQ;
+  constexpr int EPB = 4 * TPW;
+  const int blocks = (K + EPB - 1) / EPB;
+  update_kernel<S, TPW><<<blocks, WAVE * 4, 0, st>>>(K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ,
+                                                     Q, Qf, p->gamma, std::log(p->gamma - 1.0));
+  return hip_check(hipGetLastError());
+}
+
+struct Instance {
+  int dim, N, Nq, Nfq;
+  Ops ops;
+};
+
+template <int DIM, int NQ, int NFQ> constexpr Instance make_instance(int N) {
+  using S = Shape<DIM, NQ, NFQ>;
+  return Instance{DIM, N, NQ, NFQ, Ops{&vol_impl<S>, &surf_impl<S>, &update_impl<S>, S::WS}};
+}
+
+// (dim, N, Nq, Nfq): quadrature of degree 2N, the point counts of the reference's
+// tabulated rules (node_data.h) for M = 2N.
+const Instance kInstances[] = {
+    make_instance<2, 3, 2>(1),  make_instance<2, 6, 3>(2),  make_instance<2, 12, 4>(3),
+    make_instance<2, 16, 5>(4), make_instance<2, 25, 6>(5), make_instance<3, 4, 3>(1),
+    make_instance<3, 23, 12>(3), make_instance<3, 44, 16>(4),
+};
+constexpr int kNumInstances = sizeof(kInstances) / sizeof(kInstances[0]);
+
+}  // namespace wadges
+
+// ================================================================== C ABI
+
+using namespace wadges;
+
+extern "C" {
+
+int wadges_last_hip_error(void) { return g_last_hip; }
+
+const char *wadges_error_string(int code) {
+  switch (code) {
+    case WADGES_OK: return "ok";
+    case WADGES_EINVAL: return "invalid argument";
+    case WADGES_EUNSUPPORTED: return "no kernel instance for (dim, Nq, Nfq)";
+    case WADGES_EHIP: return "HIP runtime error";
+    case WADGES_ENOMEM: return "device allocation failed";
+    default: return "unknown error";
+  }
+}
+
+int wadges_instance_count(void) { return kNumInstances; }
+
+int wadges_instance(int idx, int *dim, int *N, int *Nq, int *Nfq) {
+  if (idx < 0 || idx >= kNumInstances) return WADGES_EINVAL;
+  const Instance &I = kInstances[idx];
+  if (dim) *dim = I.dim;
+  if (N) *N = I.N;
+  if (Nq) *Nq = I.Nq;
+  if (Nfq) *Nfq = I.Nfq;
+  return WADGES_OK;
+}
+
+int wadges_plan_create(wadges_plan **plan, int dim, int N, int Nq, int Nfq, double gamma,
+                       double tau) {
+  if (!plan) return WADGES_EINVAL;
+  *plan = nullptr;
+  if (!(gamma > 1.0) || !(tau >= 0.0)) return WADGES_EINVAL;
+  const Instance *inst = nullptr;
+  for (int n = 0; n < kNumInstances; ++n)
+    if (kInstances[n].dim == dim && kInstances[n].Nq == Nq && kInstances[n].Nfq == Nfq)
+      inst = &kInstances[n];
+  if (!inst) return WADGES_EUNSUPPORTED;
+  wadges_plan *p = new (std::nothrow) wadges_plan;
+  if (!p) return WADGES_ENOMEM;
+  p->dim = dim;
+  p->N = N;
+  p->Nq = Nq;
+  p->Nfq = Nfq;
+  p->Nfaces = dim + 1;
+  p->Nf = Nfq * (dim + 1);
+  p->nfields = dim + 2;
+  p->gamma = gamma;
+  p->tau = tau;
+  p->ops = inst->ops;
+  p->ws = nullptr;
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return WADGES_EHIP;
+  }
+  hipError_t e = hipMalloc(&p->ws, sizeof(double) * (size_t)inst->ops.ws_doubles);
+  if (e != hipSuccess) {
+    g_last_hip = (int)e;
+    delete p;
+    return WADGES_ENOMEM;
+  }
+  *plan = p;
+  return WADGES_OK;
+}
+
+int wadges_plan_destroy(wadges_plan *plan) {
+  if (!plan) return WADGES_OK;
+  int rc = WADGES_OK;
+  if (plan->ws) rc = hip_check(hipFree(plan->ws));
+  delete plan;
+  return rc;
+}
+
+int wadges_vol(const wadges_plan *p, int K, const double *vgeo, const double *vfgeo,
+               const double *nrJ, const double *nsJ, const double *ntJ, const double *Drq,
+               const double *Dsq, const double *Dtq, const double *Drstq, const double *VqLq,
+               const double *VfPq, const double *Q, const double *Qf, double *rhsQ,
+               double *rhsQf, void *stream) {
+  (void)Drstq;
+  (void)Q;
+  if (!p || K < 0) return WADGES_EINVAL;
+  if (K > 0 && (!vgeo || !vfgeo || !nrJ || !nsJ || !Drq || !Dsq || !VqLq || !VfPq || !Qf ||
+                !rhsQ || !rhsQf))
+    return WADGES_EINVAL;
+  if (K > 0 && p->dim == 3 && (!ntJ || !Dtq)) return WADGES_EINVAL;
+  return p->ops.vol(p, K, vgeo, vfgeo, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, VqLq, VfPq, Qf, rhsQ, rhsQf,
+                    (hipStream_t)stream);
+}
+
+int wadges_surf(const wadges_plan *p, int K, const double *fgeo, const uint32_t *mapPq,
+                const double *VqLq, const double *Qf, const double *rhsQf, double *rhsQ,
+                void *stream) {
+  if (!p || K < 0) return WADGES_EINVAL;
+  if (K > 0 && (!fgeo || !mapPq || !VqLq || !Qf || !rhsQf || !rhsQ)) return WADGES_EINVAL;
+  return p->ops.surf(p, K, fgeo, mapPq, VqLq, Qf, rhsQf, rhsQ, (hipStream_t)stream);
+}
+
+int wadges_update(const wadges_plan *p, int K, const double *Jq, const double *VqPq,
+                  const double *VfPq, double fa, double fb, double fdt, double *rhsQ,
+                  double *resQ, double *Q, double *Qf, void *stream) {
+  if (!p || K < 0) return WADGES_EINVAL;
+  if (K > 0 && (!Jq || !VqPq || !VfPq || !rhsQ || !resQ || !Q || !Qf)) return WADGES_EINVAL;
+  return p->ops.update(p, K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ, Q, Qf, (hipStream_t)stream);
+}
+
+}  // extern "C"

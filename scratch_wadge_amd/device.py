@@ -1,0 +1,123 @@
+"""Device-resident problem and the RK driver over the C ABI.
+
+The reference's rk_step / rk_run (euler.c:3610-3738) as host code: constant
+arrays are uploaded once, the state lives in HBM, and each LSRK stage is
+vol -> surf -> update launched asynchronously on one HIP stream.  PyTorch is
+used only to allocate device memory and to get the current stream; all
+arithmetic runs in libwadges.so.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .problem import RK4A, RK4B, Problem
+
+
+def _dev(a, device, dtype=torch.float64):
+    if a is None:
+        return None
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dtype)
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+class DeviceSolver:
+    CONST = ("vgeo", "vfgeo", "fgeo", "Jq", "Drq", "Dsq", "Dtq", "VqLq", "VqPq", "VfPq", "nrJ",
+             "nsJ", "ntJ")
+    STATE = ("Q", "rhsQ", "Qf", "resQ")
+
+    def __init__(self, prob: Problem, device="cuda:0"):
+        if not torch.cuda.is_available():
+            raise _lib.WadgesError("the WADGES kernels need a HIP device (torch.cuda is unavailable)")
+        self.prob = prob
+        self.device = torch.device(device)
+        self.K = prob.K
+        nq_tot = prob.K * prob.Nq * prob.nfields
+        nf_tot = prob.K * prob.Nf * prob.nfields
+        sizes = {"vgeo": prob.K * prob.Nq * prob.dim ** 2, "vfgeo": prob.K * prob.Nf * prob.dim ** 2,
+                 "fgeo": prob.K * prob.Nf * (prob.dim + 1), "Jq": prob.K * prob.Nq,
+                 "Q": nq_tot, "rhsQ": nq_tot, "resQ": nq_tot, "Qf": nf_tot}
+        for name, n in sizes.items():
+            if getattr(prob, name).size != n:
+                raise _lib.WadgesError(f"{name} has {getattr(prob, name).size} entries, expected {n}")
+        m = np.asarray(prob.mapPq)
+        if m.size != prob.K * prob.Nf or (m.size and int(m.max()) + (prob.nfields - 1) * prob.Nf >= nf_tot):
+            raise _lib.WadgesError("mapPq does not index inside Qf")
+        self.plan = _lib.Plan(prob.dim, prob.N, prob.Nq, prob.Nfq, prob.gamma, prob.tau)
+        self.c = {k: _dev(getattr(prob, k), self.device) for k in self.CONST}
+        self.c["mapPq"] = torch.from_numpy(prob.mapPq.astype(np.int32)).to(self.device)
+        self.s = {k: _dev(getattr(prob, k), self.device) for k in self.STATE}
+        self.rhsQf = torch.zeros_like(self.s["Qf"])
+
+    # -------------------------------------------------------------- state
+    def set_state(self, **arrays):
+        for k, v in arrays.items():
+            self.s[k].copy_(torch.as_tensor(np.ascontiguousarray(v), dtype=torch.float64))
+
+    def get_state(self):
+        torch.cuda.synchronize(self.device)
+        return {k: v.cpu().numpy() for k, v in self.s.items()}
+
+    def stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    # -------------------------------------------------------------- kernels
+    def vol(self, rhsQ=None, Qf=None, rhsQf=None):
+        c = self.c
+        rhsQ = self.s["rhsQ"] if rhsQ is None else rhsQ
+        Qf = self.s["Qf"] if Qf is None else Qf
+        rhsQf = self.rhsQf if rhsQf is None else rhsQf
+        self.plan.vol(self.K, _ptr(c["vgeo"]), _ptr(c["vfgeo"]), _ptr(c["nrJ"]), _ptr(c["nsJ"]),
+                      _ptr(c["ntJ"]), _ptr(c["Drq"]), _ptr(c["Dsq"]), _ptr(c["Dtq"]), None,
+                      _ptr(c["VqLq"]), _ptr(c["VfPq"]), _ptr(self.s["Q"]), _ptr(Qf), _ptr(rhsQ),
+                      _ptr(rhsQf), self.stream())
+
+    def surf(self, rhsQ=None, Qf=None, rhsQf=None):
+        c = self.c
+        rhsQ = self.s["rhsQ"] if rhsQ is None else rhsQ
+        Qf = self.s["Qf"] if Qf is None else Qf
+        rhsQf = self.rhsQf if rhsQf is None else rhsQf
+        self.plan.surf(self.K, _ptr(c["fgeo"]), _ptr(c["mapPq"]), _ptr(c["VqLq"]), _ptr(Qf),
+                       _ptr(rhsQf), _ptr(rhsQ), self.stream())
+
+    def update(self, fa, fb, dt):
+        c, s = self.c, self.s
+        self.plan.update(self.K, _ptr(c["Jq"]), _ptr(c["VqPq"]), _ptr(c["VfPq"]), fa, fb, dt,
+                         _ptr(s["rhsQ"]), _ptr(s["resQ"]), _ptr(s["Q"]), _ptr(s["Qf"]), self.stream())
+
+    def stage(self, intrk, dt):
+        self.vol()
+        self.surf()
+        self.update(RK4A[intrk], RK4B[intrk], dt)
+
+    def step(self, dt):
+        for intrk in range(5):
+            self.stage(intrk, dt)
+
+    def run(self, nsteps, dt=None):
+        dt = self.prob.dt if dt is None else dt
+        for _ in range(nsteps):
+            self.step(dt)
+
+    def rhs(self, rhsQ_in, Qf):
+        """Volume + surface RHS of a given projected state; returns host arrays."""
+        r = _dev(rhsQ_in, self.device)
+        qf = _dev(Qf, self.device)
+        rf = torch.zeros_like(qf)
+        self.vol(r, qf, rf)
+        self.surf(r, qf, rf)
+        torch.cuda.synchronize(self.device)
+        return r.cpu().numpy(), rf.cpu().numpy()
+
+    def vol_only(self, rhsQ_in, Qf):
+        r = _dev(rhsQ_in, self.device)
+        qf = _dev(Qf, self.device)
+        rf = torch.zeros_like(qf)
+        self.vol(r, qf, rf)
+        torch.cuda.synchronize(self.device)
+        return r.cpu().numpy(), rf.cpu().numpy()

@@ -1,0 +1,72 @@
+"""Algorithmic work model of one RK stage (flops and compulsory HBM bytes).
+
+Flops count + - * / and one per log/exp/pow/sqrt, per element and stage.  The
+per-piece constants are exact operation counts of the reference's formulas,
+obtained by instantiating the CPU oracle's templated flux / contraction /
+node-conversion routines with a counting scalar type (tests check that these
+constants still equal those counts):
+
+    c_F    one two-point entropy-conservative flux from node records
+           (okl/Euler3D.okl:122-162, okl/Euler2D.okl:139-161)
+    c_node conservative -> (rho, u, beta) conversion of one node
+    c_vv   averaged-metric contraction of one ordered volume-volume pair
+           (okl/Euler3D.okl:2317-2340)
+    c_vf   averaged-normal contraction of one volume-face pair, both coupling
+           blocks (okl/Euler3D.okl:2383-2403 and :2488-2507)
+
+Volume kernel work counts each distinct flux once (the EC flux is symmetric):
+
+    F_vol = P_F c_F + Nq^2 c_vv + Nq Nf c_vf + (Nq + Nf) c_node,
+    P_F   = Nq (Nq + 1)/2 + Nq Nf.
+
+Compulsory FP64 traffic of the three-kernel dataflow (bytes per element):
+
+    B_vol  = 8 (14 (Nq + Nf) + 5 (Nq + Nf))      state + geofacs in, rhs out
+    B_surf = 8 (19 Nf + 10 Nq) + 4 Nf             traces, fgeo, rhsQf, rhsQ RMW, mapPq
+    B_upd  = 8 (31 Nq + 5 Nf)                     rhs, res, Q, J in; res, Q, rhsQ, Qf out
+"""
+
+from __future__ import annotations
+
+# exact counts (add+mul+div+log), see module docstring
+PIECES = {
+    3: {"c_F": 75, "c_node": 15, "c_vv": 63, "c_vf": 138},
+    2: {"c_F": 63, "c_node": 11, "c_vv": 48, "c_vf": 106},
+}
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+
+
+def vol_flops(dim, Nq, Nf):
+    p = PIECES[dim]
+    PF = Nq * (Nq + 1) // 2 + Nq * Nf
+    return PF * p["c_F"] + Nq * Nq * p["c_vv"] + Nq * Nf * p["c_vf"] + (Nq + Nf) * p["c_node"]
+
+
+def vol_flops_reference_structure(dim, Nq, Nf):
+    """Work of the reference kernel's loop structure: Nq^2 + 2 Nq Nf flux calls, each
+    converting its partner node (okl/Euler3D.okl:2263-2273)."""
+    p = PIECES[dim]
+    nflux = Nq * Nq + 2 * Nq * Nf
+    return nflux * (p["c_F"] + p["c_node"]) + Nq * Nq * p["c_vv"] + Nq * Nf * p["c_vf"]
+
+
+def bytes_vol(dim, Nq, Nf):
+    nfld = dim + 2
+    return 8 * ((nfld + dim * dim) + nfld) * (Nq + Nf)
+
+
+def bytes_surf(dim, Nq, Nf):
+    nfld = dim + 2
+    return 8 * ((3 * nfld + dim + 1) * Nf + 2 * nfld * Nq) + 4 * Nf
+
+
+def bytes_update(dim, Nq, Nf):
+    nfld = dim + 2
+    return 8 * ((6 * nfld + 1) * Nq + nfld * Nf)
+
+
+def dofs(dim, N, K):
+    Np = (N + 1) * (N + 2) // 2 if dim == 2 else (N + 1) * (N + 2) * (N + 3) // 6
+    return K * Np * (dim + 2)
