@@ -50,6 +50,7 @@ def lib():
                                 ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
         L.wo_flop_counts.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.wo_set_threads.argtypes = [ctypes.c_int]
+        L.wo_set_logmean_variant.argtypes = [ctypes.c_int]
         L.wo_get_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -68,6 +69,11 @@ def params(prob):
 
 def set_threads(n):
     lib().wo_set_threads(int(n))
+
+
+def set_logmean_variant(v):
+    """0 = intended log mean; 1 = the reference's as-shipped 3D log branch (tests only)."""
+    lib().wo_set_logmean_variant(int(v))
 
 
 def logmean(dim, a, b):

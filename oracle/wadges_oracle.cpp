@@ -89,6 +89,11 @@ inline FlopCount wmax(FlopCount a, FlopCount b) { g_tally.cmp++; return a.v > b.
 
 // ----------------------------------------------------------------- pointwise math
 
+// 0: intended log mean.  1: the log branch as shipped in okl/Euler3D.okl:68,
+// (aL+aR)/(ln(xi) f), which is off by 1/f^2.  Variant 1 exists only so the tests
+// can plant that mistake in the oracle and show that their checks reject it.
+int g_logmean_variant = 0;
+
 // Log mean (aL - aR)/(ln aL - ln aR).
 template <int DIM, typename T> inline T logmean(T aL, T aR) {
   if constexpr (DIM == 3) {
@@ -101,6 +106,7 @@ template <int DIM, typename T> inline T logmean(T aL, T aR) {
       const T F = T(105.0) + T(35.0) * u + T(21.0) * u2 + T(15.0) * u2 * u;
       return T(52.5) * (aL + aR) / F;
     }
+    if (g_logmean_variant == 1) return (aL + aR) / (wlog(xi) * f);
     return (aL + aR) * f / wlog(xi);
   } else {
     // okl/Euler2D.okl:54-66.
@@ -541,6 +547,9 @@ typedef struct {
 } wo_params;
 
 void wo_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
+// Test hook: 0 = intended 3D log mean, 1 = the as-shipped log branch (see logmean).
+void wo_set_logmean_variant(int v) { g_logmean_variant = v; }
 
 int wo_get_max_threads(void) {
 #ifdef _OPENMP

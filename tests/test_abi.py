@@ -1,0 +1,53 @@
+"""The C-ABI library loads and exports what include/wadges.h declares (no GPU calls)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from scratch_wadge_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "wadges.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(wadges_[a-z_]+)\s*\(", src)))
+
+
+def test_header_matches_binding_list():
+    assert declared_symbols() == sorted(_lib.EXPORTED)
+
+
+def test_library_exports_every_symbol():
+    L = _lib.load()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+
+
+def test_instances_and_error_strings():
+    inst = _lib.instances()
+    assert (3, 3, 23, 12) in inst and (3, 4, 44, 16) in inst and (2, 3, 12, 4) in inst
+    L = _lib.load()
+    for code in (0, -1, -2, -3, -4):
+        assert L.wadges_error_string(code)
+    # argument checks run before any HIP call
+    rc = L.wadges_vol(None, 10, *([None] * 15), None)
+    assert rc == -1
+    h = ctypes.c_void_p()
+    assert L.wadges_plan_create(ctypes.byref(h), 3, 3, 24, 12, 1.4, 1.0) == -2  # no such instance
+    assert L.wadges_plan_create(ctypes.byref(h), 3, 3, 23, 12, 0.9, 1.0) == -1  # gamma <= 1
+
+
+def test_workmodel_matches_oracle_flop_counts(oracle):
+    from scratch_wadge_amd import workmodel as W
+    for dim, Nq, Nfq in ((3, 23, 12), (2, 12, 4)):
+        c = oracle.flop_counts(dim, Nq, Nfq)
+        for k in ("c_F", "c_node", "c_vv", "c_vf"):
+            assert W.PIECES[dim][k] == c[k], (dim, k)
+    assert W.vol_flops(3, 23, 48) == 290244
+    assert W.bytes_vol(3, 23, 48) == 10792
+    assert W.bytes_surf(3, 23, 48) == 9328
+    assert W.bytes_update(3, 23, 48) == 7624
