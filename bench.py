@@ -35,7 +35,23 @@ def parse():
     ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="instead of the benchmark, copy a GIB-sized device buffer 5 times "
+                         "(reference kernel of known traffic for the HBM counters)")
     return ap.parse_args()
+
+
+def copy_calibration(gib):
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_copy": n * 8,
+                                            "bytes_written_per_copy": n * 8, "copies": 5}}))
 
 
 def cpu_baseline(prob, target_s):
@@ -82,6 +98,9 @@ def cpu_baseline(prob, target_s):
 
 def main():
     args = parse()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the element "
               f"partition across GPUs is designed but not built)", file=sys.stderr)

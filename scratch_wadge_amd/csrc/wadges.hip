@@ -63,13 +63,45 @@ template <int DIM_, int NQ_, int NFQ_> struct Shape {
   static constexpr int WS = WS_D + WS_W;
 };
 
+// ------------------------------------------------------------------ pair schedules
+//
+// Volume-volume block: lane i visits j = (i + t) mod NQ, t = 0..NQ/2 (for even NQ
+// only lanes i < NQ/2 take t = NQ/2), so every unordered pair and the diagonal
+// is evaluated once, and at fixed t the partners j are distinct.
+//
+// Volume-face block, "face-chunked" when NQ <= NFN: lanes form groups of NFQ
+// (g = i / NFQ, o = i % NFQ); step t = c*NFQ + m visits face (c + g) mod NFACES,
+// point (o + m) mod NFQ.  Within a step the partners are distinct (distinct
+// faces across groups, distinct points within a group) and within a chunk c a
+// lane's face is fixed, so its own-side normal term is computed once per face.
+// Otherwise (2D N >= 4, where NQ > NFN) the plain diagonal fn = (i + t) mod L.
+template <class S> __host__ __device__ constexpr bool face_chunked() { return S::NQ <= S::NFN; }
+
+template <class S> __device__ __forceinline__ int vf_partner(int t, int i) {
+  if constexpr (face_chunked<S>()) {
+    const int c = t / S::NFQ, m = t - (t / S::NFQ) * S::NFQ;
+    const int g = i / S::NFQ, o = i - (i / S::NFQ) * S::NFQ;
+    int face = c + g;
+    if (face >= S::NFACES) face -= S::NFACES;
+    int node = o + m;
+    if (node >= S::NFQ) node -= S::NFQ;
+    return face * S::NFQ + node;
+  } else {
+    int f = i + t;
+    if (f >= S::L) f -= S::L;
+    return f;  // >= NFN means "no pair at this step"
+  }
+}
+
+template <class S> constexpr int vf_steps() { return face_chunked<S>() ? S::NFN : S::L; }
+
 // ------------------------------------------------------------------ operator prep
 
 template <class S>
 __global__ void prep_vol_tables(const double *__restrict__ Drq, const double *__restrict__ Dsq,
                                 const double *__restrict__ Dtq, const double *__restrict__ VqLq,
                                 const double *__restrict__ VfPq, double *__restrict__ ws) {
-  constexpr int NQ = S::NQ, NFN = S::NFN, L = S::L, TVT = S::TVT, DIM = S::DIM;
+  constexpr int NQ = S::NQ, NFN = S::NFN, TVT = S::TVT, DIM = S::DIM, T = vf_steps<S>();
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < S::WS_D) {
     const int i = idx % NQ;
@@ -80,13 +112,12 @@ __global__ void prep_vol_tables(const double *__restrict__ Drq, const double *__
     if (j >= NQ) j -= NQ;
     const double *D = r == 0 ? Drq : (r == 1 ? Dsq : Dtq);
     ws[idx] = s == 0 ? D[i + j * NQ] : D[j + i * NQ];
-  } else if (idx < S::WS) {
+  } else if (idx < S::WS_D + 2 * T * NQ) {
     const int k = idx - S::WS_D;
     const int i = k % NQ;
-    const int t = (k / NQ) % L;
-    const int s = k / (NQ * L);
-    int f = i + t;
-    if (f >= L) f -= L;
+    const int t = (k / NQ) % T;
+    const int s = k / (NQ * T);
+    const int f = vf_partner<S>(t, i);
     double v = 0.0;
     if (f < NFN) v = s == 0 ? 0.5 * VqLq[i + f * NQ] : -0.5 * VfPq[f + i * NFN];
     ws[idx] = v;
@@ -95,10 +126,37 @@ __global__ void prep_vol_tables(const double *__restrict__ Drq, const double *__
 
 // ------------------------------------------------------------------ volume kernel
 
+// LDS per element: the volume accumulator, then one region used first for the
+// volume-node records (volume-volume block) and then for the face-node records
+// and face accumulator (volume-face block).
 template <class S> struct VolSmem {
-  static constexpr int per_elem =
-      S::VREC * S::NQ + S::FREC * S::NFN + S::NFLD * S::NQ + S::NFLD * S::NFN;
+  static constexpr int acc_v = S::NFLD * S::NQ;
+  static constexpr int phase1 = S::VREC * S::NQ;
+  static constexpr int phase2 = (S::FREC + S::NFLD) * S::NFN;
+  static constexpr int per_elem = acc_v + (phase1 > phase2 ? phase1 : phase2);
 };
+
+template <int DIM>
+__device__ __forceinline__ Node<DIM> load_node(const double *rec, int stride, int j) {
+  Node<DIM> n;
+  n.rho = rec[0 * stride + j];
+  for (int d = 0; d < DIM; ++d) n.hv[d] = rec[(1 + d) * stride + j];
+  n.beta = rec[(DIM + 1) * stride + j];
+  n.lrho = rec[(DIM + 2) * stride + j];
+  n.lbeta = rec[(DIM + 3) * stride + j];
+  n.hq = rec[(DIM + 4) * stride + j];
+  return n;
+}
+
+template <int DIM>
+__device__ __forceinline__ void store_node(double *rec, int stride, int j, const Node<DIM> &n) {
+  rec[0 * stride + j] = n.rho;
+  for (int d = 0; d < DIM; ++d) rec[(1 + d) * stride + j] = n.hv[d];
+  rec[(DIM + 1) * stride + j] = n.beta;
+  rec[(DIM + 2) * stride + j] = n.lrho;
+  rec[(DIM + 3) * stride + j] = n.lbeta;
+  rec[(DIM + 4) * stride + j] = n.hq;
+}
 
 template <class S, int TPW>
 __global__ void __launch_bounds__(WAVE)
@@ -108,8 +166,9 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
            const double *__restrict__ Qf, double *rhsQ, double *__restrict__ rhsQf,
            double gm1, double inv2gm1) {
   constexpr int DIM = S::DIM, NQ = S::NQ, NFQ = S::NFQ, NFN = S::NFN, NFLD = S::NFLD;
-  constexpr int NG = S::NG, L = S::L, TVT = S::TVT, VS = S::VS, VREC = S::VREC, FREC = S::FREC;
+  constexpr int NG = S::NG, TVT = S::TVT, VS = S::VS, FREC = S::FREC;
   constexpr int PER = VolSmem<S>::per_elem;
+  constexpr int T = vf_steps<S>();
   extern __shared__ double smem[];
 
   const int lane = threadIdx.x;
@@ -125,52 +184,25 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
     s_nref[lane][1] = nsJ[lane * NFQ];
     if constexpr (DIM == 3) s_nref[lane][2] = ntJ[lane * NFQ];
   }
-  __syncthreads();
 
-  double *vrec = smem + (active ? team : 0) * PER;
-  double *frec = vrec + VREC * NQ;
-  double *accV = frec + FREC * NFN;
-  double *accF = accV + NFLD * NQ;
+  double *accV = smem + (active ? team : 0) * PER;
+  double *vrec = accV + NFLD * NQ;         // phase 1
+  double *frec = vrec;                      // phase 2
+  double *accF = frec + FREC * NFN;
 
+  // ---------------- phase 1: volume records, volume-volume block
   Node<DIM> me;
   double hG[NG];
   if (valid) {
-    // own volume node
     double U[NFLD];
     const double *src = rhsQ + (size_t)k * NQ * NFLD;
     for (int f = 0; f < NFLD; ++f) U[f] = src[i + f * NQ];
     me = make_node<DIM>(U, gm1);
     const double *g = vgeo + (size_t)k * NQ * NG;
     for (int c = 0; c < NG; ++c) hG[c] = 0.5 * g[i + c * NQ];
-    vrec[0 * NQ + i] = me.rho;
-    for (int d = 0; d < DIM; ++d) vrec[(1 + d) * NQ + i] = me.hv[d];
-    vrec[(DIM + 1) * NQ + i] = me.beta;
-    vrec[(DIM + 2) * NQ + i] = me.lrho;
-    vrec[(DIM + 3) * NQ + i] = me.lbeta;
-    vrec[(DIM + 4) * NQ + i] = me.hq;
+    store_node<DIM>(vrec, NQ, i, me);
     for (int c = 0; c < NG; ++c) vrec[(VS + c) * NQ + i] = hG[c];
     for (int f = 0; f < NFLD; ++f) accV[f * NQ + i] = 0.0;
-    // face nodes owned by this lane
-    const double *srcf = Qf + (size_t)k * NFN * NFLD;
-    const double *gf = vfgeo + (size_t)k * NFN * NG;
-    for (int fn = i; fn < NFN; fn += NQ) {
-      double Uf[NFLD];
-      for (int f = 0; f < NFLD; ++f) Uf[f] = srcf[fn + f * NFN];
-      const Node<DIM> nf = make_node<DIM>(Uf, gm1);
-      frec[0 * NFN + fn] = nf.rho;
-      for (int d = 0; d < DIM; ++d) frec[(1 + d) * NFN + fn] = nf.hv[d];
-      frec[(DIM + 1) * NFN + fn] = nf.beta;
-      frec[(DIM + 2) * NFN + fn] = nf.lrho;
-      frec[(DIM + 3) * NFN + fn] = nf.lbeta;
-      frec[(DIM + 4) * NFN + fn] = nf.hq;
-      const int face = fn / NFQ;
-      for (int d = 0; d < DIM; ++d) {
-        double b = 0.0;
-        for (int r = 0; r < DIM; ++r) b = b + 0.5 * gf[fn + (r * DIM + d) * NFN] * s_nref[face][r];
-        frec[(VS + d) * NFN + fn] = b;
-      }
-      for (int f = 0; f < NFLD; ++f) accF[f * NFN + fn] = 0.0;
-    }
   }
   __syncthreads();
 
@@ -178,7 +210,6 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
   for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
 
   if (valid) {
-    // ---------------- volume-volume block: pairs {i, (i+t) mod NQ}, t = 0..NQ/2
     const double *Dfw = ws;
     const double *Dbw = ws + DIM * TVT * NQ;
     for (int t = 0; t < TVT; ++t) {
@@ -186,13 +217,7 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
       if (2 * t == NQ && i >= NQ / 2) continue;  // even NQ: middle pair once
       int j = i + t;
       if (j >= NQ) j -= NQ;
-      Node<DIM> pj;
-      pj.rho = vrec[0 * NQ + j];
-      for (int d = 0; d < DIM; ++d) pj.hv[d] = vrec[(1 + d) * NQ + j];
-      pj.beta = vrec[(DIM + 1) * NQ + j];
-      pj.lrho = vrec[(DIM + 2) * NQ + j];
-      pj.lbeta = vrec[(DIM + 3) * NQ + j];
-      pj.hq = vrec[(DIM + 4) * NQ + j];
+      const Node<DIM> pj = load_node<DIM>(vrec, NQ, j);
       double Ga[NG];
       for (int c = 0; c < NG; ++c) Ga[c] = hG[c] + vrec[(VS + c) * NQ + j];
       const PairFlux<DIM> P = pair_flux<DIM>(me, pj, inv2gm1);
@@ -215,28 +240,36 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
         for (int f = 0; f < NFLD; ++f) lds_add(&accV[f * NQ + j], c[f]);
       }
     }
+  }
+  __syncthreads();
 
-    // ---------------- volume-face block: pairs (i, (i+t) mod L), t = 0..L-1
-    const double *W1 = ws + S::WS_D;
-    const double *W2 = W1 + L * NQ;
-    for (int t = 0; t < L; ++t) {
-      int fn = i + t;
-      if (fn >= L) fn -= L;
-      if (fn >= NFN) continue;
-      Node<DIM> pf;
-      pf.rho = frec[0 * NFN + fn];
-      for (int d = 0; d < DIM; ++d) pf.hv[d] = frec[(1 + d) * NFN + fn];
-      pf.beta = frec[(DIM + 1) * NFN + fn];
-      pf.lrho = frec[(DIM + 2) * NFN + fn];
-      pf.lbeta = frec[(DIM + 3) * NFN + fn];
-      pf.hq = frec[(DIM + 4) * NFN + fn];
+  // ---------------- phase 2: face records (overwrite the volume records)
+  if (valid) {
+    const double *srcf = Qf + (size_t)k * NFN * NFLD;
+    const double *gf = vfgeo + (size_t)k * NFN * NG;
+    for (int fn = i; fn < NFN; fn += NQ) {
+      double Uf[NFLD];
+      for (int f = 0; f < NFLD; ++f) Uf[f] = srcf[fn + f * NFN];
+      store_node<DIM>(frec, NFN, fn, make_node<DIM>(Uf, gm1));
       const int face = fn / NFQ;
-      double n[DIM];
       for (int d = 0; d < DIM; ++d) {
-        double a = frec[(VS + d) * NFN + fn];
-        for (int r = 0; r < DIM; ++r) a = a + hG[r * DIM + d] * s_nref[face][r];
-        n[d] = a;
+        double b = 0.0;
+        for (int r = 0; r < DIM; ++r) b = b + 0.5 * gf[fn + (r * DIM + d) * NFN] * s_nref[face][r];
+        frec[(VS + d) * NFN + fn] = b;
       }
+      for (int f = 0; f < NFLD; ++f) accF[f * NFN + fn] = 0.0;
+    }
+  }
+  __syncthreads();
+
+  if (valid) {
+    const double *W1 = ws + S::WS_D;
+    const double *W2 = W1 + T * NQ;
+    // one volume-face pair: partner face point fn, own-side normal part a
+    auto vf_pair = [&](int t, int fn, const double (&a)[DIM]) {
+      const Node<DIM> pf = load_node<DIM>(frec, NFN, fn);
+      double n[DIM];
+      for (int d = 0; d < DIM; ++d) n[d] = a[d] + frec[(VS + d) * NFN + fn];
       const PairFlux<DIM> P = pair_flux<DIM>(me, pf, inv2gm1);
       double Fn[NFLD];
       flux_dot<DIM>(P, n, Fn);
@@ -245,6 +278,36 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
       for (int f = 0; f < NFLD; ++f) {
         acc[f] += w1 * Fn[f];
         lds_add(&accF[f * NFN + fn], w2 * Fn[f]);
+      }
+    };
+    // own-side part of the averaged normal on a face: sum_r (G_rd / 2) nref_r
+    auto own_normal = [&](int face, double (&a)[DIM]) {
+      for (int d = 0; d < DIM; ++d) {
+        double v = 0.0;
+        for (int r = 0; r < DIM; ++r) v = v + hG[r * DIM + d] * s_nref[face][r];
+        a[d] = v;
+      }
+    };
+    if constexpr (face_chunked<S>()) {
+      const int g = i / NFQ, o = i - (i / NFQ) * NFQ;
+      for (int c = 0; c < S::NFACES; ++c) {
+        int face = c + g;
+        if (face >= S::NFACES) face -= S::NFACES;
+        double a[DIM];
+        own_normal(face, a);
+        for (int m = 0; m < NFQ; ++m) {
+          int node = o + m;
+          if (node >= NFQ) node -= NFQ;
+          vf_pair(c * NFQ + m, face * NFQ + node, a);
+        }
+      }
+    } else {
+      for (int t = 0; t < T; ++t) {
+        const int fn = vf_partner<S>(t, i);
+        if (fn >= NFN) continue;
+        double a[DIM];
+        own_normal(fn / NFQ, a);
+        vf_pair(t, fn, a);
       }
     }
   }

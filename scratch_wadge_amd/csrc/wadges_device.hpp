@@ -40,6 +40,16 @@ template <int DIM> struct Node {
   double hq;       // |u|^2 / 2
 };
 
+// 1/x from v_rcp_f64 refined by two Newton steps (the first half of the compiler's
+// IEEE division sequence, without the final quotient correction): ~1 ulp.
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
 // Conservative state -> node record.  3D multiplies by 1/rho
 // (okl/Euler3D.okl:2252-2256, TESTING); 2D divides (okl/Euler2D.okl:1127-1129).
 template <int DIM>
@@ -81,7 +91,8 @@ template <int DIM> __device__ __forceinline__ double logmean_series(double u) {
     return 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0))));
 }
 
-// inv2gm1 = 1 / (2 (gamma - 1)).
+// inv2gm1 = 1 / (2 (gamma - 1)).  Both log-mean branches are evaluated and
+// selected without branching (lanes of a wave take different branches).
 template <int DIM>
 __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Node<DIM> &b,
                                                    double inv2gm1) {
@@ -89,18 +100,22 @@ __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Nod
   const double sr = a.rho + b.rho, dr = a.rho - b.rho;
   const double sb = a.beta + b.beta, db = a.beta - b.beta;
   // f = (aL-aR)/(aL+aR) for both log means from one reciprocal
-  const double q1 = 1.0 / (sr * sb);
+  const double q1 = rcp(sr * sb);
   const double fr = dr * sb * q1;
   const double fb = db * sr * q1;
   const double ur = fr * fr, ub = fb * fb;
+  const double polyR = 2.0 * logmean_series<DIM>(ur);
+  const double polyB = 2.0 * logmean_series<DIM>(ub);
+  const double ldR = a.lrho - b.lrho;
+  const double ldB = a.lbeta - b.lbeta;
   const bool serR = ur < TH, serB = ub < TH;
   // logmean = num / den:  series (aL+aR) / (2 F(u)),  log (aL-aR) / (ln aL - ln aR)
   const double numR = serR ? sr : dr;
-  const double denR = serR ? 2.0 * logmean_series<DIM>(ur) : (a.lrho - b.lrho);
+  const double denR = serR ? polyR : ldR;
   const double numB = serB ? sb : db;
-  const double denB = serB ? 2.0 * logmean_series<DIM>(ub) : (a.lbeta - b.lbeta);
+  const double denB = serB ? polyB : ldB;
   // rholog = numR/denR, rholog/betalog = numR denB/(denR numB), pa = (sr/2)/sb
-  const double q = 1.0 / (denR * numB * sb);
+  const double q = rcp(denR * numB * sb);
   const double A = numB * q;
   PairFlux<DIM> P;
   P.rholog = numR * sb * A;

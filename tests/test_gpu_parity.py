@@ -69,25 +69,44 @@ def test_update_parity(oracle, dim, N):
         assert relerr(g[k], ref[k]) < TOL, k
 
 
+def _oracle_run(oracle, prob, steps, noise=0.0, seed=0):
+    """LSRK45 in the oracle; noise > 0 multiplies every RHS by (1 + noise*N(0,1))
+    before the update, which measures how round-off in the RHS propagates."""
+    from scratch_wadge_amd.problem import RK4A, RK4B
+    rng = np.random.default_rng(seed)
+    st = prob.copy_state()
+    for _ in range(steps):
+        for s in range(5):
+            rf = np.zeros_like(st["Qf"])
+            oracle.vol(prob, st["rhsQ"], st["Qf"], rf)
+            oracle.surf(prob, st["Qf"], rf, st["rhsQ"])
+            if noise:
+                st["rhsQ"] *= 1.0 + noise * rng.standard_normal(st["rhsQ"].shape)
+            oracle.update(prob, RK4A[s], RK4B[s], prob.dt, st["rhsQ"], st["resQ"], st["Q"], st["Qf"])
+    return st
+
+
 @pytest.mark.parametrize("dim,N", [(2, 3), (3, 3)])
 def test_rk_steps_parity(oracle, dim, N):
-    """Five full LSRK45 steps (25 stages) from the projected initial state."""
+    """Five full LSRK45 steps (25 stages) from the projected initial state.
+
+    The solution, the projected state and the face traces must agree to 1e-12.
+    The RK register resQ is a small difference of large stage increments: its
+    round-off floor is measured by rerunning the oracle with 1e-15 relative
+    noise on every RHS, and the GPU must stay within 10x of that floor."""
     prob = small_problem(dim, N)
     print(f"case rk dim={dim} N={N} K={prob.K}", flush=True)
     dev = _solver(prob)
     dev.run(5)
     g = dev.get_state()
-    from scratch_wadge_amd.problem import RK4A, RK4B
-    st = prob.copy_state()
-    for _ in range(5):
-        for s in range(5):
-            oracle.stage(prob, st, RK4A[s], RK4B[s], prob.dt)
-    # each RHS evaluation agrees to <= 1e-12; 25 stages accumulate round-off
-    # through the state (resQ is a difference of nearly equal stage increments)
+    st = _oracle_run(oracle, prob, 5)
+    floor = _oracle_run(oracle, prob, 5, noise=1e-15)
     errs = {k: relerr(g[k], st[k]) for k in ("Q", "resQ", "rhsQ", "Qf")}
-    print("relative L2 after 25 stages:", errs, flush=True)
-    for k, e in errs.items():
-        assert e < 1e-10, k
+    floors = {k: relerr(floor[k], st[k]) for k in errs}
+    print("relative L2 after 25 stages:", errs, "round-off floor:", floors, flush=True)
+    for k in ("Q", "rhsQ", "Qf"):
+        assert errs[k] < 1e-12, k
+    assert errs["resQ"] < max(10 * floors["resQ"], 1e-12)
 
 
 def test_deterministic(oracle):
