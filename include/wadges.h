@@ -88,6 +88,66 @@ int wadges_update(const wadges_plan *plan, int K, const double *Jq, const double
                   const double *VfPq, double fa, double fb, double fdt, double *rhsQ,
                   double *resQ, double *Q, double *Qf, void *stream);
 
+/* One whole LSRK stage in a single launch: vol + surf + update fused per
+ * element (no counterpart kernel in the reference; it replaces the three
+ * launches of rk_step, euler.c:3656-3666, with one).  The volume RHS stays on
+ * chip.  Because neighbouring elements read face traces of the previous stage
+ * while an element writes its new ones, the face traces are double-buffered:
+ * the stage reads Qf_in and writes Qf_out (must differ; swap them between
+ * stages).  rhsQ (projected volume state in and out), resQ and Q are updated
+ * in place as by wadges_update.  Results equal vol -> surf -> update to
+ * round-off. */
+int wadges_stage(const wadges_plan *plan, int K, const double *vgeo, const double *vfgeo,
+                 const double *fgeo, const uint32_t *mapPq, const double *nrJ, const double *nsJ,
+                 const double *ntJ, const double *Drq, const double *Dsq, const double *Dtq,
+                 const double *VqLq, const double *VqPq, const double *VfPq, const double *Jq,
+                 double fa, double fb, double fdt, const double *Qf_in, double *Qf_out,
+                 double *rhsQ, double *resQ, double *Q, void *stream);
+
+/* ---- Discrete adjoint (vector-Jacobian products); no counterpart in the
+ * reference.  Cotangent arrays use the layout of their primal arrays.
+ *
+ * wadges_rhs_vjp: given rhsQ_bar, the cotangent of the stage RHS (the rhsQ that
+ * vol -> surf produce from the projected state rhsQ and traces Qf), writes the
+ * cotangents of that state: rhsQ_state_bar [K][Nfields][Nq] and Qf_bar
+ * [K][Nfields][Nf].  sbar is caller scratch of K*Nfields*Nf doubles.  The log
+ * mean is differentiated in the branch the forward takes; max(lamM, lamP) of
+ * the Lax-Friedrichs penalty passes its derivative to the larger argument (to
+ * the element's own point at ties). */
+int wadges_rhs_vjp(const wadges_plan *plan, int K, const double *vgeo, const double *vfgeo,
+                   const double *fgeo, const uint32_t *mapPq, const double *nrJ, const double *nsJ,
+                   const double *ntJ, const double *Drq, const double *Dsq, const double *Dtq,
+                   const double *VqLq, const double *VfPq, const double *rhsQ, const double *Qf,
+                   const double *rhsQ_bar, double *sbar, double *rhsQ_state_bar,
+                   double *Qf_bar, void *stream);
+
+/* wadges_update_vjp: VJP of wadges_update.  Q is the state AFTER the update.
+ * On entry Q_bar, resQ_bar hold the cotangents of the updated Q, resQ and
+ * rhsQ_bar_out, Qf_bar_out those of the updated projected state; on return
+ * Q_bar, resQ_bar hold the cotangents of Q, resQ before the update and rhsQ_bar
+ * that of the RHS the update consumed. */
+int wadges_update_vjp(const wadges_plan *plan, int K, const double *Jq, const double *VqPq,
+                      const double *VfPq, double fa, double fb, double fdt, const double *Q,
+                      double *Q_bar, double *resQ_bar, const double *rhsQ_bar_out,
+                      const double *Qf_bar_out, double *rhsQ_bar, void *stream);
+
+/* wadges_project_vjp: Q_bar += VJP of the initial projection
+ * rhsQ = u(VqPq v(Q)), Qf = u(VfPq v(Q)) (euler.c:4028-4133) at Q. */
+int wadges_project_vjp(const wadges_plan *plan, int K, const double *VqPq, const double *VfPq,
+                       const double *Q, const double *rhsQ_bar, const double *Qf_bar,
+                       double *Q_bar, void *stream);
+
+/* Diagnostics as device reductions (the reference computes them on the host or
+ * in euler_3d_compute_aux, okl/Euler3D.okl:3888-3934, euler.c:4135-4156 and
+ * :4227-4274).  Writes per element aux[K][4] = (kinetic energy
+ * sum wq J |m|^2/(2 rho), entropy sum wq J (-rho s), squared L2 error
+ * sum wq J |Q - Qexact|^2 (0 if Qexact is NULL), mass sum wq J rho), and their
+ * sums over the elements to totals[4], in a fixed summation order.  aux is
+ * caller-provided scratch of K*4 doubles. */
+int wadges_diagnostics(const wadges_plan *plan, int K, const double *wq, const double *Jq,
+                       const double *Q, const double *Qexact, double *aux, double *totals,
+                       void *stream);
+
 /* Introspection. */
 int wadges_last_hip_error(void);           /* hipError_t of the last failing HIP call */
 const char *wadges_error_string(int code);  /* text for a WADGES_E* code */

@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "libwadges.so")
 # Every symbol the header declares (checked by the CPU tests).
 EXPORTED = (
     "wadges_plan_create", "wadges_plan_destroy", "wadges_vol", "wadges_surf", "wadges_update",
+    "wadges_stage", "wadges_diagnostics", "wadges_rhs_vjp", "wadges_update_vjp",
+    "wadges_project_vjp",
     "wadges_last_hip_error", "wadges_error_string", "wadges_instance_count", "wadges_instance",
 )
 
@@ -41,6 +43,12 @@ def load():
     L.wadges_surf.argtypes = [_vp, ctypes.c_int] + [_vp] * 6 + [_vp]
     L.wadges_update.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, _vp, _vp, _vp, _vp, _vp]
+    L.wadges_stage.argtypes = [_vp, ctypes.c_int] + [_vp] * 14 + [ctypes.c_double] * 3 + [_vp] * 5 + [_vp]
+    L.wadges_diagnostics.argtypes = [_vp, ctypes.c_int] + [_vp] * 6 + [_vp]
+    L.wadges_rhs_vjp.argtypes = [_vp, ctypes.c_int] + [_vp] * 18 + [_vp]
+    L.wadges_update_vjp.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double] + [_vp] * 6 + [_vp]
+    L.wadges_project_vjp.argtypes = [_vp, ctypes.c_int] + [_vp] * 6 + [_vp]
     L.wadges_error_string.restype = ctypes.c_char_p
     L.wadges_error_string.argtypes = [ctypes.c_int]
     L.wadges_instance.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4
@@ -93,6 +101,31 @@ class Plan:
 
     def surf(self, K, fgeo, mapPq, VqLq, Qf, rhsQf, rhsQ, stream=None):
         check(load().wadges_surf(self.h, K, fgeo, mapPq, VqLq, Qf, rhsQf, rhsQ, stream), "wadges_surf")
+
+    def stage(self, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, VqLq, VqPq, VfPq, Jq,
+              fa, fb, fdt, Qf_in, Qf_out, rhsQ, resQ, Q, stream=None):
+        check(load().wadges_stage(self.h, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq,
+                                  VqLq, VqPq, VfPq, Jq, fa, fb, fdt, Qf_in, Qf_out, rhsQ, resQ, Q,
+                                  stream), "wadges_stage")
+
+    def diagnostics(self, K, wq, Jq, Q, Qexact, aux, totals, stream=None):
+        check(load().wadges_diagnostics(self.h, K, wq, Jq, Q, Qexact, aux, totals, stream),
+              "wadges_diagnostics")
+
+    def rhs_vjp(self, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, VqLq, VfPq, rhsQ, Qf,
+                rhsQ_bar, sbar, uq_bar, uf_bar, stream=None):
+        check(load().wadges_rhs_vjp(self.h, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq,
+                                    VqLq, VfPq, rhsQ, Qf, rhsQ_bar, sbar, uq_bar, uf_bar, stream),
+              "wadges_rhs_vjp")
+
+    def update_vjp(self, K, Jq, VqPq, VfPq, fa, fb, fdt, Q, Q_bar, res_bar, uq_bar_out, uf_bar_out,
+                   rhsQ_bar, stream=None):
+        check(load().wadges_update_vjp(self.h, K, Jq, VqPq, VfPq, fa, fb, fdt, Q, Q_bar, res_bar,
+                                       uq_bar_out, uf_bar_out, rhsQ_bar, stream), "wadges_update_vjp")
+
+    def project_vjp(self, K, VqPq, VfPq, Q, uq_bar, uf_bar, Q_bar, stream=None):
+        check(load().wadges_project_vjp(self.h, K, VqPq, VfPq, Q, uq_bar, uf_bar, Q_bar, stream),
+              "wadges_project_vjp")
 
     def update(self, K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ, Q, Qf, stream=None):
         check(load().wadges_update(self.h, K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ, Q, Qf, stream),

@@ -158,34 +158,19 @@ __device__ __forceinline__ void store_node(double *rec, int stride, int j, const
   rec[(DIM + 4) * stride + j] = n.hq;
 }
 
-template <class S, int TPW>
-__global__ void __launch_bounds__(WAVE)
-vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vfgeo,
-           const double *__restrict__ nrJ, const double *__restrict__ nsJ,
-           const double *__restrict__ ntJ, const double *__restrict__ ws,
-           const double *__restrict__ Qf, double *rhsQ, double *__restrict__ rhsQf,
-           double gm1, double inv2gm1) {
+// The volume term of one element (both coupling blocks).  On return acc holds
+// the register part of volume row i, accV the LDS part (partner contributions),
+// accF the face rows (rhsQf).  Uq is the element's projected volume state,
+// Uf its face state.  Every lane of the workgroup must call this (barriers).
+template <class S>
+__device__ __forceinline__ void vol_body(bool valid, int i, const double *Uq, const double *Uf,
+                                         const double *vg, const double *vfg,
+                                         const double (*s_nref)[S::DIM], const double *ws,
+                                         double *accV, double gm1, double inv2gm1,
+                                         double (&acc)[S::NFLD]) {
   constexpr int DIM = S::DIM, NQ = S::NQ, NFQ = S::NFQ, NFN = S::NFN, NFLD = S::NFLD;
   constexpr int NG = S::NG, TVT = S::TVT, VS = S::VS, FREC = S::FREC;
-  constexpr int PER = VolSmem<S>::per_elem;
   constexpr int T = vf_steps<S>();
-  extern __shared__ double smem[];
-
-  const int lane = threadIdx.x;
-  const int team = lane / NQ;
-  const int i = lane - team * NQ;
-  const bool active = team < TPW;
-  const int k = blockIdx.x * TPW + team;
-  const bool valid = active && k < K;
-
-  __shared__ double s_nref[S::NFACES][DIM];
-  if (lane < S::NFACES) {
-    s_nref[lane][0] = nrJ[lane * NFQ];
-    s_nref[lane][1] = nsJ[lane * NFQ];
-    if constexpr (DIM == 3) s_nref[lane][2] = ntJ[lane * NFQ];
-  }
-
-  double *accV = smem + (active ? team : 0) * PER;
   double *vrec = accV + NFLD * NQ;         // phase 1
   double *frec = vrec;                      // phase 2
   double *accF = frec + FREC * NFN;
@@ -195,18 +180,15 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
   double hG[NG];
   if (valid) {
     double U[NFLD];
-    const double *src = rhsQ + (size_t)k * NQ * NFLD;
-    for (int f = 0; f < NFLD; ++f) U[f] = src[i + f * NQ];
+    for (int f = 0; f < NFLD; ++f) U[f] = Uq[i + f * NQ];
     me = make_node<DIM>(U, gm1);
-    const double *g = vgeo + (size_t)k * NQ * NG;
-    for (int c = 0; c < NG; ++c) hG[c] = 0.5 * g[i + c * NQ];
+    for (int c = 0; c < NG; ++c) hG[c] = 0.5 * vg[i + c * NQ];
     store_node<DIM>(vrec, NQ, i, me);
     for (int c = 0; c < NG; ++c) vrec[(VS + c) * NQ + i] = hG[c];
     for (int f = 0; f < NFLD; ++f) accV[f * NQ + i] = 0.0;
   }
   __syncthreads();
 
-  double acc[NFLD];
   for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
 
   if (valid) {
@@ -245,16 +227,14 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
 
   // ---------------- phase 2: face records (overwrite the volume records)
   if (valid) {
-    const double *srcf = Qf + (size_t)k * NFN * NFLD;
-    const double *gf = vfgeo + (size_t)k * NFN * NG;
     for (int fn = i; fn < NFN; fn += NQ) {
-      double Uf[NFLD];
-      for (int f = 0; f < NFLD; ++f) Uf[f] = srcf[fn + f * NFN];
-      store_node<DIM>(frec, NFN, fn, make_node<DIM>(Uf, gm1));
+      double U[NFLD];
+      for (int f = 0; f < NFLD; ++f) U[f] = Uf[fn + f * NFN];
+      store_node<DIM>(frec, NFN, fn, make_node<DIM>(U, gm1));
       const int face = fn / NFQ;
       for (int d = 0; d < DIM; ++d) {
         double b = 0.0;
-        for (int r = 0; r < DIM; ++r) b = b + 0.5 * gf[fn + (r * DIM + d) * NFN] * s_nref[face][r];
+        for (int r = 0; r < DIM; ++r) b = b + 0.5 * vfg[fn + (r * DIM + d) * NFN] * s_nref[face][r];
         frec[(VS + d) * NFN + fn] = b;
       }
       for (int f = 0; f < NFLD; ++f) accF[f * NFN + fn] = 0.0;
@@ -312,6 +292,45 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
     }
   }
   __syncthreads();
+
+}
+
+template <class S>
+__device__ __forceinline__ void load_nref(int lane, const double *nrJ, const double *nsJ,
+                                          const double *ntJ, double (*s_nref)[S::DIM]) {
+  if (lane < S::NFACES) {
+    s_nref[lane][0] = nrJ[lane * S::NFQ];
+    s_nref[lane][1] = nsJ[lane * S::NFQ];
+    if constexpr (S::DIM == 3) s_nref[lane][2] = ntJ[lane * S::NFQ];
+  }
+}
+
+template <class S, int TPW>
+__global__ void __launch_bounds__(WAVE)
+vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vfgeo,
+           const double *__restrict__ nrJ, const double *__restrict__ nsJ,
+           const double *__restrict__ ntJ, const double *__restrict__ ws,
+           const double *__restrict__ Qf, double *rhsQ, double *__restrict__ rhsQf,
+           double gm1, double inv2gm1) {
+  constexpr int NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD, NG = S::NG, FREC = S::FREC;
+  constexpr int PER = VolSmem<S>::per_elem;
+  extern __shared__ double smem[];
+  __shared__ double s_nref[S::NFACES][S::DIM];
+
+  const int lane = threadIdx.x;
+  const int team = lane / NQ;
+  const int i = lane - team * NQ;
+  const bool active = team < TPW;
+  const int k = blockIdx.x * TPW + team;
+  const bool valid = active && k < K;
+  const size_t kk = valid ? (size_t)k : 0;
+
+  load_nref<S>(lane, nrJ, nsJ, ntJ, s_nref);
+  double *accV = smem + (active ? team : 0) * PER;
+  double acc[NFLD];
+  vol_body<S>(valid, i, rhsQ + kk * NQ * NFLD, Qf + kk * NFN * NFLD, vgeo + kk * NQ * NG,
+              vfgeo + kk * NFN * NG, s_nref, ws, accV, gm1, inv2gm1, acc);
+  const double *accF = accV + NFLD * NQ + FREC * NFN;
 
   if (valid) {
     double *dst = rhsQ + (size_t)k * NQ * NFLD;
@@ -504,6 +523,230 @@ update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ V
   }
 }
 
+// ------------------------------------------------------------------ fused stage kernel
+//
+// One LSRK stage for one element per team in a single launch: volume term,
+// surface term (neighbour traces through mapPq) and its lift, weight-adjusted
+// inverse mass, RK update and entropy projection.  The RHS never leaves the
+// chip.  Neighbours read face traces of the previous stage while this element
+// writes its new ones, so the face traces are double-buffered: Qf_in is read,
+// Qf_out written (the caller swaps them between stages).
+template <class S, int TPW>
+__global__ void __launch_bounds__(WAVE)
+stage_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vfgeo,
+             const double *__restrict__ fgeo, const uint32_t *__restrict__ mapPq,
+             const double *__restrict__ nrJ, const double *__restrict__ nsJ,
+             const double *__restrict__ ntJ, const double *__restrict__ ws,
+             const double *__restrict__ VqLq, const double *__restrict__ VqPq,
+             const double *__restrict__ VfPq, const double *__restrict__ Jq, double fa, double fb,
+             double fdt, const double *__restrict__ Qf_in, double *__restrict__ Qf_out,
+             double *rhsQ, double *__restrict__ resQ, double *__restrict__ Q, double gamma,
+             double gm1, double inv2gm1, double ltau, double lgm1) {
+  constexpr int DIM = S::DIM, NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD, NG = S::NG;
+  constexpr int FREC = S::FREC, NFG = DIM + 1;
+  constexpr int PER = VolSmem<S>::per_elem;
+  extern __shared__ double smem[];
+  __shared__ double s_nref[S::NFACES][DIM];
+
+  const int lane = threadIdx.x;
+  const int team = lane / NQ;
+  const int i = lane - team * NQ;
+  const bool active = team < TPW;
+  const int k = blockIdx.x * TPW + team;
+  const bool valid = active && k < K;
+  const size_t kk = valid ? (size_t)k : 0;
+  const size_t base = kk * NQ * NFLD;
+
+  load_nref<S>(lane, nrJ, nsJ, ntJ, s_nref);
+  double *accV = smem + (active ? team : 0) * PER;
+  double *accF = accV + NFLD * NQ + FREC * NFN;
+  double acc[NFLD];
+  vol_body<S>(valid, i, rhsQ + base, Qf_in + kk * NFN * NFLD, vgeo + kk * NQ * NG,
+              vfgeo + kk * NFN * NG, s_nref, ws, accV, gm1, inv2gm1, acc);
+
+  // ---------------- surface flux + LF penalty, in place over the face rows
+  if (valid) {
+    const double *g = fgeo + kk * NFN * NFG;
+    for (int fn = i; fn < NFN; fn += NQ) {
+      const size_t idM = (size_t)fn + kk * NFN * NFLD;
+      const size_t idP = mapPq[kk * NFN + fn];
+      double UM[NFLD], UP[NFLD];
+      for (int f = 0; f < NFLD; ++f) {
+        UM[f] = Qf_in[idM + (size_t)f * NFN];
+        UP[f] = Qf_in[idP + (size_t)f * NFN];
+      }
+      double nJ[DIM];
+      for (int d = 0; d < DIM; ++d) nJ[d] = g[fn + d * NFN];
+      const double sJ = g[fn + DIM * NFN];
+      const Node<DIM> a = make_node<DIM>(UM, gm1);
+      const Node<DIM> b = make_node<DIM>(UP, gm1);
+      const double pM = gm1 * (UM[NFLD - 1] - 0.5 * UM[0] * (2.0 * a.hq));
+      const double pP = gm1 * (UP[NFLD - 1] - 0.5 * UP[0] * (2.0 * b.hq));
+      const double lamM = sqrt(2.0 * a.hq) + sqrt(gamma * pM / UM[0]);
+      const double lamP = sqrt(2.0 * b.hq) + sqrt(gamma * pP / UP[0]);
+      const double Lfc = ltau * fmax(lamM, lamP) * sJ;
+      const PairFlux<DIM> P = pair_flux<DIM>(a, b, inv2gm1);
+      double Fn[NFLD];
+      flux_dot<DIM>(P, nJ, Fn);
+      for (int f = 0; f < NFLD; ++f)
+        accF[f * NFN + fn] = 0.5 * Fn[f] + accF[f * NFN + fn] - Lfc * (UP[f] - UM[f]);
+    }
+  }
+  __syncthreads();
+  // ---------------- RHS row i = volume part + lift of the face terms
+  double rhs[NFLD];
+  if (valid) {
+    double val[NFLD];
+    for (int f = 0; f < NFLD; ++f) val[f] = 0.0;
+#pragma unroll 4
+    for (int j = 0; j < NFN; ++j) {
+      const double L = VqLq[i + j * NQ];
+      for (int f = 0; f < NFLD; ++f) val[f] += L * accF[f * NFN + j];
+    }
+    for (int f = 0; f < NFLD; ++f) rhs[f] = (acc[f] + accV[f * NQ + i]) + val[f];
+  }
+  __syncthreads();
+
+  // ---------------- update (same sequence as update_kernel)
+  double *bufA = accV;                // [NFLD][NQ]
+  double *bufB = accV + NFLD * NQ;    // [NFLD][NQ]
+  double J = 1.0, invJ = 1.0;
+  if (valid) {
+    J = Jq[kk * NQ + i];
+    invJ = 1.0 / J;
+    for (int f = 0; f < NFLD; ++f) bufA[f * NQ + i] = rhs[f];
+  }
+  __syncthreads();
+  auto matvec = [&](const double *in, double scale, double (&r)[NFLD]) {
+    for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
+#pragma unroll 2
+    for (int j = 0; j < NQ; ++j) {
+      const double P = scale * VqPq[i + j * NQ];
+      for (int f = 0; f < NFLD; ++f) r[f] += P * in[f * NQ + j];
+    }
+  };
+  if (valid) {
+    double r[NFLD];
+    matvec(bufA, -2.0, r);
+    for (int f = 0; f < NFLD; ++f) bufB[f * NQ + i] = (DIM == 3) ? r[f] * invJ : r[f] / J;
+  }
+  __syncthreads();
+  if (valid) {
+    double r[NFLD], U[NFLD], V[NFLD];
+    matvec(bufB, 1.0, r);
+    for (int f = 0; f < NFLD; ++f) {
+      const size_t id = base + i + f * NQ;
+      const double res = fa * resQ[id] + fdt * r[f];
+      resQ[id] = res;
+      U[f] = Q[id] + fb * res;
+      Q[id] = U[f];
+    }
+    entropy_vars<DIM>(U, gamma, V);
+    for (int f = 0; f < NFLD; ++f) bufA[f * NQ + i] = V[f] * J;
+  }
+  __syncthreads();
+  if (valid) {
+    double r[NFLD];
+    matvec(bufA, 1.0, r);
+    for (int f = 0; f < NFLD; ++f) bufB[f * NQ + i] = (DIM == 3) ? r[f] * invJ : r[f] / J;
+  }
+  __syncthreads();
+  if (valid) {
+    double r[NFLD];
+    matvec(bufB, 1.0, r);
+    for (int f = 0; f < NFLD; ++f) bufA[f * NQ + i] = r[f];
+  }
+  __syncthreads();
+  if (valid) {
+    double *qf = Qf_out + kk * NFN * NFLD;
+    for (int fn = i; fn < NFN; fn += NQ) {
+      double V[NFLD], U[NFLD];
+      for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
+#pragma unroll 2
+      for (int j = 0; j < NQ; ++j) {
+        const double P = VfPq[fn + j * NFN];
+        for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[f * NQ + j];
+      }
+      conservative_vars<DIM>(V, gamma, lgm1, U);
+      for (int f = 0; f < NFLD; ++f) qf[fn + f * NFN] = U[f];
+    }
+    double V[NFLD], U[NFLD];
+    matvec(bufA, 1.0, V);
+    conservative_vars<DIM>(V, gamma, lgm1, U);
+    for (int f = 0; f < NFLD; ++f) rhsQ[base + i + f * NQ] = U[f];
+  }
+}
+
+// ------------------------------------------------------------------ diagnostics
+//
+// Per element: aux[k][0] = sum wq J |m|^2/(2 rho)      (euler_3d_compute_aux, okl/Euler3D.okl:3888-3934)
+//              aux[k][1] = sum wq J (-rho s)           (entropy, euler.c:4135-4156)
+//              aux[k][2] = sum wq J |Q - Qex|^2        (squared L2 error, euler.c:4227-4274)
+//              aux[k][3] = sum wq J rho                (mass)
+// then totals[c] = sum_k aux[k][c] by one workgroup in a fixed order.
+constexpr int NDIAG = 4;
+
+template <class S, int TPW>
+__global__ void __launch_bounds__(WAVE)
+diag_kernel(int K, const double *__restrict__ wq, const double *__restrict__ Jq,
+            const double *__restrict__ Q, const double *__restrict__ Qex, double *__restrict__ aux,
+            double gamma) {
+  constexpr int DIM = S::DIM, NQ = S::NQ, NFLD = S::NFLD;
+  __shared__ double part[TPW][NDIAG][NQ];
+  const int lane = threadIdx.x;
+  const int team = lane / NQ;
+  const int i = lane - team * NQ;
+  const bool active = team < TPW;
+  const int k = blockIdx.x * TPW + team;
+  const bool valid = active && k < K;
+  if (valid) {
+    const size_t id = (size_t)k * NQ * NFLD + i;
+    double U[NFLD];
+    for (int f = 0; f < NFLD; ++f) U[f] = Q[id + f * NQ];
+    const double wJ = wq[i] * Jq[(size_t)k * NQ + i];
+    double mm = U[1] * U[1];
+    for (int d = 1; d < DIM; ++d) mm = mm + U[1 + d] * U[1 + d];
+    const double rhoe = U[NFLD - 1] - 0.5 * mm / U[0];
+    const double s = log((gamma - 1.0) * rhoe) - gamma * log(U[0]);
+    double err = 0.0;
+    if (Qex) {
+      for (int f = 0; f < NFLD; ++f) {
+        const double e = U[f] - Qex[id + f * NQ];
+        err = err + e * e;
+      }
+    }
+    part[team][0][i] = 0.5 * mm / U[0] * wJ;
+    part[team][1][i] = wJ * (-U[0] * s);
+    part[team][2][i] = err * wJ;
+    part[team][3][i] = wJ * U[0];
+  }
+  __syncthreads();
+  if (valid && i < NDIAG) {
+    double t = 0.0;
+    for (int j = 0; j < NQ; ++j) t += part[team][i][j];
+    aux[(size_t)k * NDIAG + i] = t;
+  }
+}
+
+__global__ void __launch_bounds__(1024)
+diag_reduce_kernel(int K, const double *__restrict__ aux, double *__restrict__ totals) {
+  __shared__ double red[NDIAG][1024];
+  const int t = threadIdx.x;
+  double acc[NDIAG] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = t; k < K; k += 1024)
+    for (int c = 0; c < NDIAG; ++c) acc[c] += aux[(size_t)k * NDIAG + c];
+  for (int c = 0; c < NDIAG; ++c) red[c][t] = acc[c];
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w)
+      for (int c = 0; c < NDIAG; ++c) red[c][t] += red[c][t + w];
+    __syncthreads();
+  }
+  if (t < NDIAG) totals[t] = red[t][0];
+}
+
+#include "wadges_adjoint.hpp"
+
 // ------------------------------------------------------------------ dispatch
 
 thread_local int g_last_hip = 0;
@@ -524,6 +767,22 @@ struct Ops {
               const double *, const double *, double *, hipStream_t);
   int (*update)(const wadges_plan *, int, const double *, const double *, const double *,
                 double, double, double, double *, double *, double *, double *, hipStream_t);
+  int (*stage)(const wadges_plan *, int, const double *, const double *, const double *,
+               const uint32_t *, const double *, const double *, const double *, const double *,
+               const double *, const double *, const double *, const double *, const double *,
+               const double *, double, double, double, const double *, double *, double *,
+               double *, double *, hipStream_t);
+  int (*diag)(const wadges_plan *, int, const double *, const double *, const double *,
+              const double *, double *, double *, hipStream_t);
+  int (*rhs_vjp)(const wadges_plan *, int, const double *, const double *, const double *,
+                 const uint32_t *, const double *, const double *, const double *, const double *,
+                 const double *, const double *, const double *, const double *, const double *,
+                 const double *, const double *, double *, double *, double *, hipStream_t);
+  int (*update_vjp)(const wadges_plan *, int, const double *, const double *, const double *,
+                    double, double, double, const double *, double *, double *, const double *,
+                    const double *, double *, hipStream_t);
+  int (*proj_vjp)(const wadges_plan *, int, const double *, const double *, const double *,
+                  const double *, const double *, double *, hipStream_t);
   int ws_doubles;
 };
 
@@ -590,6 +849,93 @@ template <class S> int update_impl(const wadges_plan *p, int K, const double *Jq
   return hip_check(hipGetLastError());
 }
 
+template <class S> int stage_impl(const wadges_plan *p, int K, const double *vgeo,
+                                  const double *vfgeo, const double *fgeo, const uint32_t *mapPq,
+                                  const double *nrJ, const double *nsJ, const double *ntJ,
+                                  const double *Drq, const double *Dsq, const double *Dtq,
+                                  const double *VqLq, const double *VqPq, const double *VfPq,
+                                  const double *Jq, double fa, double fb, double fdt,
+                                  const double *Qf_in, double *Qf_out, double *rhsQ, double *resQ,
+                                  double *Q, hipStream_t st) {
+  if (K == 0) return WADGES_OK;
+  prep_vol_tables<S><<<(S::WS + 255) / 256, 256, 0, st>>>(Drq, Dsq, Dtq, VqLq, VfPq, p->ws);
+  int rc = hip_check(hipGetLastError());
+  if (rc) return rc;
+  constexpr int TPW = WAVE / S::NQ;
+  const size_t smem = (size_t)TPW * VolSmem<S>::per_elem * sizeof(double);
+  const int blocks = (K + TPW - 1) / TPW;
+  const double g = p->gamma;
+  stage_kernel<S, TPW><<<blocks, WAVE, smem, st>>>(
+      K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, p->ws, VqLq, VqPq, VfPq, Jq, fa, fb, fdt, Qf_in,
+      Qf_out, rhsQ, resQ, Q, g, g - 1.0, 1.0 / (2.0 * (g - 1.0)), 0.25 * p->tau, std::log(g - 1.0));
+  return hip_check(hipGetLastError());
+}
+
+template <class S> int diag_impl(const wadges_plan *p, int K, const double *wq, const double *Jq,
+                                 const double *Q, const double *Qex, double *aux, double *totals,
+                                 hipStream_t st) {
+  constexpr int TPW = WAVE / S::NQ;
+  if (K > 0) {
+    diag_kernel<S, TPW><<<(K + TPW - 1) / TPW, WAVE, 0, st>>>(K, wq, Jq, Q, Qex, aux, p->gamma);
+    int rc = hip_check(hipGetLastError());
+    if (rc) return rc;
+  }
+  diag_reduce_kernel<<<1, 1024, 0, st>>>(K, aux, totals);
+  return hip_check(hipGetLastError());
+}
+
+template <class S> int rhs_vjp_impl(const wadges_plan *p, int K, const double *vgeo,
+                                    const double *vfgeo, const double *fgeo, const uint32_t *mapPq,
+                                    const double *nrJ, const double *nsJ, const double *ntJ,
+                                    const double *Drq, const double *Dsq, const double *Dtq,
+                                    const double *VqLq, const double *VfPq, const double *Uq,
+                                    const double *Uf, const double *Rb, double *sbar, double *uqb,
+                                    double *ufb, hipStream_t st) {
+  if (K == 0) return WADGES_OK;
+  prep_vol_tables<S><<<(S::WS + 255) / 256, 256, 0, st>>>(Drq, Dsq, Dtq, VqLq, VfPq, p->ws);
+  int rc = hip_check(hipGetLastError());
+  if (rc) return rc;
+  {
+    constexpr int TS = Max<S::NQ, S::NFN>::value;
+    constexpr int TPW = WAVE / TS;
+    constexpr int EPB = 4 * TPW;
+    sbar_kernel<S, TS, TPW><<<(K + EPB - 1) / EPB, WAVE * 4, 0, st>>>(K, VqLq, Rb, sbar);
+    rc = hip_check(hipGetLastError());
+    if (rc) return rc;
+  }
+  constexpr int TPW = WAVE / S::NQ;
+  const size_t smem = (size_t)TPW * VjpSmem<S>::per_elem * sizeof(double);
+  const double g = p->gamma;
+  rhs_vjp_kernel<S, TPW><<<(K + TPW - 1) / TPW, WAVE, smem, st>>>(
+      K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, p->ws, Uq, Uf, Rb, sbar, uqb, ufb, g, g - 1.0,
+      1.0 / (2.0 * (g - 1.0)), 0.25 * p->tau);
+  return hip_check(hipGetLastError());
+}
+
+template <class S> int update_vjp_impl(const wadges_plan *p, int K, const double *Jq,
+                                       const double *VqPq, const double *VfPq, double fa, double fb,
+                                       double fdt, const double *Q, double *Qb, double *resb,
+                                       const double *uqb, const double *ufb, double *Rb,
+                                       hipStream_t st) {
+  if (K == 0) return WADGES_OK;
+  constexpr int TPW = WAVE / S::NQ;
+  constexpr int EPB = 4 * TPW;
+  update_vjp_kernel<S, TPW><<<(K + EPB - 1) / EPB, WAVE * 4, 0, st>>>(
+      K, Jq, VqPq, VfPq, fa, fb, fdt, Q, Qb, resb, uqb, ufb, Rb, p->gamma, std::log(p->gamma - 1.0));
+  return hip_check(hipGetLastError());
+}
+
+template <class S> int proj_vjp_impl(const wadges_plan *p, int K, const double *VqPq,
+                                     const double *VfPq, const double *Q, const double *uqb,
+                                     const double *ufb, double *Qb, hipStream_t st) {
+  if (K == 0) return WADGES_OK;
+  constexpr int TPW = WAVE / S::NQ;
+  constexpr int EPB = 4 * TPW;
+  proj_vjp_kernel<S, TPW><<<(K + EPB - 1) / EPB, WAVE * 4, 0, st>>>(
+      K, VqPq, VfPq, Q, uqb, ufb, Qb, p->gamma, std::log(p->gamma - 1.0));
+  return hip_check(hipGetLastError());
+}
+
 struct Instance {
   int dim, N, Nq, Nfq;
   Ops ops;
@@ -597,7 +943,9 @@ struct Instance {
 
 template <int DIM, int NQ, int NFQ> constexpr Instance make_instance(int N) {
   using S = Shape<DIM, NQ, NFQ>;
-  return Instance{DIM, N, NQ, NFQ, Ops{&vol_impl<S>, &surf_impl<S>, &update_impl<S>, S::WS}};
+  return Instance{DIM, N, NQ, NFQ,
+                  Ops{&vol_impl<S>, &surf_impl<S>, &update_impl<S>, &stage_impl<S>, &diag_impl<S>,
+                      &rhs_vjp_impl<S>, &update_vjp_impl<S>, &proj_vjp_impl<S>, S::WS}};
 }
 
 // (dim, N, Nq, Nfq): quadrature of degree 2N, the point counts of the reference's
@@ -717,6 +1065,65 @@ int wadges_update(const wadges_plan *p, int K, const double *Jq, const double *V
   if (!p || K < 0) return WADGES_EINVAL;
   if (K > 0 && (!Jq || !VqPq || !VfPq || !rhsQ || !resQ || !Q || !Qf)) return WADGES_EINVAL;
   return p->ops.update(p, K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ, Q, Qf, (hipStream_t)stream);
+}
+
+int wadges_diagnostics(const wadges_plan *p, int K, const double *wq, const double *Jq,
+                       const double *Q, const double *Qexact, double *aux, double *totals,
+                       void *stream) {
+  if (!p || K < 0 || !aux || !totals) return WADGES_EINVAL;
+  if (K > 0 && (!wq || !Jq || !Q)) return WADGES_EINVAL;
+  return p->ops.diag(p, K, wq, Jq, Q, Qexact, aux, totals, (hipStream_t)stream);
+}
+
+int wadges_rhs_vjp(const wadges_plan *p, int K, const double *vgeo, const double *vfgeo,
+                   const double *fgeo, const uint32_t *mapPq, const double *nrJ, const double *nsJ,
+                   const double *ntJ, const double *Drq, const double *Dsq, const double *Dtq,
+                   const double *VqLq, const double *VfPq, const double *rhsQ, const double *Qf,
+                   const double *rhsQ_bar, double *sbar, double *rhsQ_state_bar,
+                   double *Qf_bar, void *stream) {
+  if (!p || K < 0) return WADGES_EINVAL;
+  if (K > 0 && (!vgeo || !vfgeo || !fgeo || !mapPq || !nrJ || !nsJ || !Drq || !Dsq || !VqLq ||
+                !VfPq || !rhsQ || !Qf || !rhsQ_bar || !sbar || !rhsQ_state_bar || !Qf_bar))
+    return WADGES_EINVAL;
+  if (K > 0 && p->dim == 3 && (!ntJ || !Dtq)) return WADGES_EINVAL;
+  return p->ops.rhs_vjp(p, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, VqLq, VfPq,
+                        rhsQ, Qf, rhsQ_bar, sbar, rhsQ_state_bar, Qf_bar, (hipStream_t)stream);
+}
+
+int wadges_update_vjp(const wadges_plan *p, int K, const double *Jq, const double *VqPq,
+                      const double *VfPq, double fa, double fb, double fdt, const double *Q,
+                      double *Q_bar, double *resQ_bar, const double *rhsQ_bar_out,
+                      const double *Qf_bar_out, double *rhsQ_bar, void *stream) {
+  if (!p || K < 0) return WADGES_EINVAL;
+  if (K > 0 && (!Jq || !VqPq || !VfPq || !Q || !Q_bar || !resQ_bar || !rhsQ_bar_out ||
+                !Qf_bar_out || !rhsQ_bar))
+    return WADGES_EINVAL;
+  return p->ops.update_vjp(p, K, Jq, VqPq, VfPq, fa, fb, fdt, Q, Q_bar, resQ_bar, rhsQ_bar_out,
+                           Qf_bar_out, rhsQ_bar, (hipStream_t)stream);
+}
+
+int wadges_project_vjp(const wadges_plan *p, int K, const double *VqPq, const double *VfPq,
+                       const double *Q, const double *rhsQ_bar, const double *Qf_bar,
+                       double *Q_bar, void *stream) {
+  if (!p || K < 0) return WADGES_EINVAL;
+  if (K > 0 && (!VqPq || !VfPq || !Q || !rhsQ_bar || !Qf_bar || !Q_bar)) return WADGES_EINVAL;
+  return p->ops.proj_vjp(p, K, VqPq, VfPq, Q, rhsQ_bar, Qf_bar, Q_bar, (hipStream_t)stream);
+}
+
+int wadges_stage(const wadges_plan *p, int K, const double *vgeo, const double *vfgeo,
+                 const double *fgeo, const uint32_t *mapPq, const double *nrJ, const double *nsJ,
+                 const double *ntJ, const double *Drq, const double *Dsq, const double *Dtq,
+                 const double *VqLq, const double *VqPq, const double *VfPq, const double *Jq,
+                 double fa, double fb, double fdt, const double *Qf_in, double *Qf_out,
+                 double *rhsQ, double *resQ, double *Q, void *stream) {
+  if (!p || K < 0) return WADGES_EINVAL;
+  if (K > 0 && (!vgeo || !vfgeo || !fgeo || !mapPq || !nrJ || !nsJ || !Drq || !Dsq || !VqLq ||
+                !VqPq || !VfPq || !Jq || !Qf_in || !Qf_out || !rhsQ || !resQ || !Q))
+    return WADGES_EINVAL;
+  if (K > 0 && p->dim == 3 && (!ntJ || !Dtq)) return WADGES_EINVAL;
+  if (K > 0 && Qf_in == Qf_out) return WADGES_EINVAL;  // neighbours read Qf_in concurrently
+  return p->ops.stage(p, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, VqLq, VqPq,
+                      VfPq, Jq, fa, fb, fdt, Qf_in, Qf_out, rhsQ, resQ, Q, (hipStream_t)stream);
 }
 
 }  // extern "C"

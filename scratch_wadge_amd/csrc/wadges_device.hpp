@@ -185,3 +185,212 @@ __device__ __forceinline__ void lds_add(double *p, double v) {
 }
 
 }  // namespace wadges
+
+// ======================================================================
+// Reverse mode (discrete adjoint) building blocks.  Hand-written VJPs of the
+// forward routines above; derivatives are taken of the mathematical functions
+// the forward kernels evaluate (the log mean through whichever branch the
+// forward takes, max(lamM, lamP) through its larger argument, M at ties).
+// ======================================================================
+
+namespace wadges {
+
+// Cotangent of a node's primitive quantities (rho, velocity u, beta).
+template <int DIM> struct PrimBar {
+  double rho, u[DIM], beta;
+};
+
+template <int DIM> __device__ __forceinline__ void primbar_zero(PrimBar<DIM> &g) {
+  g.rho = 0.0;
+  for (int d = 0; d < DIM; ++d) g.u[d] = 0.0;
+  g.beta = 0.0;
+}
+
+// Partial derivatives (dl/da, dl/db) of the log mean l(a, b), in the branch the
+// forward evaluation uses: series l = (a+b)/(2F(f^2)), f = (a-b)/(a+b), or
+// l = (a-b)/(ln a - ln b).
+template <int DIM>
+__device__ __forceinline__ void logmean_partials(double a, double b, double la, double lb,
+                                                 double &l, double &da, double &db) {
+  constexpr double TH = (DIM == 3) ? 1.0e-4 : 1.0e-3;
+  const double s = a + b, d = a - b;
+  const double is = rcp(s);
+  const double f = d * is;
+  const double u = f * f;
+  if (u < TH) {
+    const double F = logmean_series<DIM>(u);
+    double Fp;  // F'(u)
+    if constexpr (DIM == 3)
+      Fp = 1.0 / 3.0 + u * (2.0 / 5.0 + u * (3.0 / 7.0));
+    else
+      Fp = 1.0 / 3.0 + u * (2.0 / 5.0 + u * (3.0 / 7.0 + u * (4.0 / 9.0)));
+    const double iF = rcp(F);
+    l = 0.5 * s * iF;
+    const double t = 2.0 * f * Fp * is * iF * iF;
+    da = 0.5 * iF - t * b;
+    db = 0.5 * iF + t * a;
+  } else {
+    const double iL = rcp(la - lb);
+    l = d * iL;
+    da = (1.0 - l * rcp(a)) * iL;
+    db = (l * rcp(b) - 1.0) * iL;
+  }
+}
+
+// VJP of L = sum_d sum_fld Gb[d][fld] F_d[fld](a, b) with respect to the
+// primitive quantities of both nodes; accumulates into ga, gb.
+template <int DIM>
+__device__ __forceinline__ void flux_vjp(const Node<DIM> &a, const Node<DIM> &b,
+                                         const double (&Gb)[DIM][DIM + 2], double inv2gm1,
+                                         PrimBar<DIM> &ga, PrimBar<DIM> &gb) {
+  constexpr int NF = DIM + 2;
+  double rl, rla, rlb, bl, bla, blb;
+  logmean_partials<DIM>(a.rho, b.rho, a.lrho, b.lrho, rl, rla, rlb);
+  logmean_partials<DIM>(a.beta, b.beta, a.lbeta, b.lbeta, bl, bla, blb);
+  const double sr = a.rho + b.rho, sb = a.beta + b.beta;
+  const double isb = rcp(sb);
+  const double pa = 0.5 * sr * isb;
+  double va[DIM], vv = 0.0;
+  for (int d = 0; d < DIM; ++d) {
+    va[d] = a.hv[d] + b.hv[d];
+    vv = vv + va[d] * va[d];
+  }
+  const double vnavg = 2.0 * vv - (a.hq + b.hq);
+  const double ibl = rcp(bl);
+  const double f4 = rl * inv2gm1 * ibl + pa + 0.5 * rl * vnavg;
+  // contractions of the cotangent with the flux structure
+  double A = 0.0, B = 0.0, tr = 0.0, C = 0.0;
+  for (int d = 0; d < DIM; ++d) {
+    A = A + Gb[d][0] * va[d];
+    C = C + Gb[d][NF - 1] * va[d];
+    tr = tr + Gb[d][1 + d];
+    for (int m = 0; m < DIM; ++m) B = B + Gb[d][1 + m] * va[d] * va[m];
+  }
+  const double L_rl = A + B + C * (inv2gm1 * ibl + 0.5 * vnavg);
+  const double L_bl = -C * rl * inv2gm1 * ibl * ibl;
+  const double L_pa = tr + C;
+  // d/d(va_d): the va enter F directly and through vnavg = 2|va|^2 - ...
+  for (int d = 0; d < DIM; ++d) {
+    double t = Gb[d][0];
+    for (int m = 0; m < DIM; ++m) t = t + (Gb[d][1 + m] + Gb[m][1 + d]) * va[m];
+    const double L_va = rl * t + f4 * Gb[d][NF - 1] + C * 2.0 * rl * va[d];
+    // va = (u_a + u_b)/2; vnavg also holds -(|u_a|^2 + |u_b|^2)/2
+    const double ua = 2.0 * a.hv[d], ub = 2.0 * b.hv[d];
+    ga.u[d] += 0.5 * L_va - 0.5 * C * rl * ua;
+    gb.u[d] += 0.5 * L_va - 0.5 * C * rl * ub;
+  }
+  ga.rho += L_rl * rla + L_pa * 0.5 * isb;
+  gb.rho += L_rl * rlb + L_pa * 0.5 * isb;
+  ga.beta += L_bl * bla - L_pa * pa * isb;
+  gb.beta += L_bl * blb - L_pa * pa * isb;
+}
+
+// Same for a rank-one cotangent Gb[d][fld] = n[d] c[fld] (the contracted
+// volume-face and surface couplings).
+template <int DIM>
+__device__ __forceinline__ void flux_dot_vjp(const Node<DIM> &a, const Node<DIM> &b,
+                                             const double (&n)[DIM], const double (&c)[DIM + 2],
+                                             double inv2gm1, PrimBar<DIM> &ga, PrimBar<DIM> &gb) {
+  double Gb[DIM][DIM + 2];
+  for (int d = 0; d < DIM; ++d)
+    for (int f = 0; f < DIM + 2; ++f) Gb[d][f] = n[d] * c[f];
+  flux_vjp<DIM>(a, b, Gb, inv2gm1, ga, gb);
+}
+
+// Primitive cotangent -> conservative cotangent of one node:
+// u = m / rho, beta = rho / (2p), p = (gamma - 1)(E - |m|^2/(2 rho)).
+template <int DIM>
+__device__ __forceinline__ void primbar_to_cons(const Node<DIM> &n, const PrimBar<DIM> &g,
+                                                double gm1, double (&Ub)[DIM + 2]) {
+  const double irho = rcp(n.rho);
+  const double p = 0.5 * n.rho * rcp(n.beta);      // p = rho / (2 beta)
+  const double bp = n.beta * rcp(p);                // beta / p
+  double gu_u = 0.0;
+  for (int d = 0; d < DIM; ++d) gu_u = gu_u + g.u[d] * (2.0 * n.hv[d]);
+  // d beta/d rho = beta/rho - (beta/p)(gamma-1)|u|^2/2
+  Ub[0] = g.rho - gu_u * irho + g.beta * (n.beta * irho - bp * gm1 * n.hq);
+  for (int d = 0; d < DIM; ++d) Ub[1 + d] = g.u[d] * irho + g.beta * bp * gm1 * (2.0 * n.hv[d]);
+  Ub[DIM + 1] = -g.beta * bp * gm1;
+}
+
+// Wave speed lambda = |u| + sqrt(gamma p / rho) and its gradient with respect
+// to the conservative state.
+template <int DIM>
+__device__ __forceinline__ double wavespeed(const double (&U)[DIM + 2], double gamma, double gm1,
+                                            double (&dl)[DIM + 2]) {
+  const double irho = rcp(U[0]);
+  double v[DIM], uu = 0.0;
+  for (int d = 0; d < DIM; ++d) {
+    v[d] = U[1 + d] * irho;
+    uu = uu + v[d] * v[d];
+  }
+  const double p = gm1 * (U[DIM + 1] - 0.5 * U[0] * uu);
+  const double su = sqrt(uu);
+  const double c = sqrt(gamma * p * irho);
+  // d|u|: d(|u|)/du_d = u_d/|u| (0 at rest); u_d = m_d/rho
+  const double isu = su > 0.0 ? rcp(su) : 0.0;
+  // c^2 = gamma p / rho: d(c)/dq = (gamma/(2c)) d(p/rho)/dq
+  const double k = 0.5 * gamma * rcp(c);
+  // p/rho = gm1 (E/rho - |m|^2/(2 rho^2))
+  const double dpr_rho = gm1 * (-U[DIM + 1] * irho * irho + uu * irho);
+  dl[0] = -uu * isu * irho + k * dpr_rho;
+  for (int d = 0; d < DIM; ++d) dl[1 + d] = v[d] * isu * irho + k * (-gm1 * v[d] * irho);
+  dl[DIM + 1] = k * gm1 * irho;
+  return su + c;
+}
+
+// VJP of entropy_vars (U -> V) at U.
+template <int DIM>
+__device__ __forceinline__ void entropy_vars_vjp(const double (&U)[DIM + 2], double gamma,
+                                                 const double (&Vb)[DIM + 2], double (&Ub)[DIM + 2]) {
+  constexpr int NF = DIM + 2;
+  const double rho = U[0], E = U[NF - 1];
+  double mm = 0.0;
+  for (int d = 0; d < DIM; ++d) mm = mm + U[1 + d] * U[1 + d];
+  const double irho = rcp(rho);
+  const double rhoe = E - 0.5 * mm * irho;
+  const double io = rcp(rhoe);
+  // V0 = -E io + (gamma + 1) - s ; V_m = m io ; V_last = -rho io
+  double iob = Vb[0] * (-E) + Vb[NF - 1] * (-rho);
+  for (int d = 0; d < DIM; ++d) iob = iob + Vb[1 + d] * U[1 + d];
+  const double sb = -Vb[0];
+  double rhob = -Vb[NF - 1] * io - sb * gamma * irho;     // s = ln((g-1) rho e) - g ln rho
+  double Eb = -Vb[0] * io;
+  const double rhoeb = -iob * io * io + sb * io;
+  Eb = Eb + rhoeb;
+  rhob = rhob + rhoeb * 0.5 * mm * irho * irho;
+  Ub[0] = rhob;
+  for (int d = 0; d < DIM; ++d) Ub[1 + d] = Vb[1 + d] * io - rhoeb * U[1 + d] * irho;
+  Ub[NF - 1] = Eb;
+}
+
+// VJP of conservative_vars (V -> U) at V.
+template <int DIM>
+__device__ __forceinline__ void conservative_vars_vjp(const double (&V)[DIM + 2], double gamma,
+                                                      double lgm1, const double (&Ub)[DIM + 2],
+                                                      double (&Vb)[DIM + 2]) {
+  constexpr int NF = DIM + 2;
+  const double vl = V[NF - 1];
+  double vv = 0.0;
+  for (int d = 0; d < DIM; ++d) vv = vv + V[1 + d] * V[1 + d];
+  const double ivl = rcp(vl);
+  const double h = 0.5 * vv * ivl;
+  const double s = gamma - V[0] + h;
+  const double igm1 = 1.0 / (gamma - 1.0);
+  const double rhoe = exp((lgm1 - gamma * log(-vl) - s) * igm1);
+  double rhoeb = Ub[0] * (-vl) + Ub[NF - 1] * (1.0 - h);
+  for (int d = 0; d < DIM; ++d) rhoeb = rhoeb + Ub[1 + d] * V[1 + d];
+  double vlb = -Ub[0] * rhoe;
+  double hb = -Ub[NF - 1] * rhoe;
+  const double Xb = rhoeb * rhoe;
+  const double sb = -Xb * igm1;
+  vlb = vlb - Xb * gamma * igm1 * ivl;
+  Vb[0] = -sb;
+  hb = hb + sb;
+  const double vvb = 0.5 * hb * ivl;
+  vlb = vlb - hb * h * ivl;
+  for (int d = 0; d < DIM; ++d) Vb[1 + d] = Ub[1 + d] * rhoe + 2.0 * vvb * V[1 + d];
+  Vb[NF - 1] = vlb;
+}
+
+}  // namespace wadges

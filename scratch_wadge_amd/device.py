@@ -53,6 +53,7 @@ class DeviceSolver:
         self.c["mapPq"] = torch.from_numpy(prob.mapPq.astype(np.int32)).to(self.device)
         self.s = {k: _dev(getattr(prob, k), self.device) for k in self.STATE}
         self.rhsQf = torch.zeros_like(self.s["Qf"])
+        self.Qf_alt = torch.zeros_like(self.s["Qf"])   # second face-trace buffer (fused stage)
 
     # -------------------------------------------------------------- state
     def set_state(self, **arrays):
@@ -91,18 +92,42 @@ class DeviceSolver:
                          _ptr(s["rhsQ"]), _ptr(s["resQ"]), _ptr(s["Q"]), _ptr(s["Qf"]), self.stream())
 
     def stage(self, intrk, dt):
+        """One LSRK stage as the reference runs it: vol -> surf -> update."""
         self.vol()
         self.surf()
         self.update(RK4A[intrk], RK4B[intrk], dt)
 
-    def step(self, dt):
-        for intrk in range(5):
-            self.stage(intrk, dt)
+    def stage_fused(self, intrk, dt):
+        """One LSRK stage in one launch (wadges_stage); swaps the face-trace buffers."""
+        c, s = self.c, self.s
+        self.plan.stage(self.K, _ptr(c["vgeo"]), _ptr(c["vfgeo"]), _ptr(c["fgeo"]), _ptr(c["mapPq"]),
+                        _ptr(c["nrJ"]), _ptr(c["nsJ"]), _ptr(c["ntJ"]), _ptr(c["Drq"]), _ptr(c["Dsq"]),
+                        _ptr(c["Dtq"]), _ptr(c["VqLq"]), _ptr(c["VqPq"]), _ptr(c["VfPq"]), _ptr(c["Jq"]),
+                        RK4A[intrk], RK4B[intrk], dt, _ptr(s["Qf"]), _ptr(self.Qf_alt), _ptr(s["rhsQ"]),
+                        _ptr(s["resQ"]), _ptr(s["Q"]), self.stream())
+        s["Qf"], self.Qf_alt = self.Qf_alt, s["Qf"]
 
-    def run(self, nsteps, dt=None):
+    def step(self, dt, fused=False):
+        for intrk in range(5):
+            (self.stage_fused if fused else self.stage)(intrk, dt)
+
+    def run(self, nsteps, dt=None, fused=False):
         dt = self.prob.dt if dt is None else dt
         for _ in range(nsteps):
-            self.step(dt)
+            self.step(dt, fused)
+
+    def diagnostics(self, Qexact=None):
+        """(kinetic energy, entropy, L2 error, mass) of the current state, reduced on
+        the device; Qexact in the Q layout (host or device array) for the error."""
+        if not hasattr(self, "_wq"):
+            self._wq = _dev(self.prob.wq, self.device)
+            self._aux = torch.empty(self.K * 4, dtype=torch.float64, device=self.device)
+            self._tot = torch.empty(4, dtype=torch.float64, device=self.device)
+        qex = None if Qexact is None else torch.as_tensor(Qexact, dtype=torch.float64).to(self.device)
+        self.plan.diagnostics(self.K, _ptr(self._wq), _ptr(self.c["Jq"]), _ptr(self.s["Q"]),
+                              _ptr(qex), _ptr(self._aux), _ptr(self._tot), self.stream())
+        ke, ent, err2, mass = self._tot.cpu().tolist()
+        return {"kinetic_energy": ke, "entropy": ent, "l2_error": err2 ** 0.5, "mass": mass}
 
     def rhs(self, rhsQ_in, Qf):
         """Volume + surface RHS of a given projected state; returns host arrays."""

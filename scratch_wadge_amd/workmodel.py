@@ -34,6 +34,16 @@ PIECES = {
     2: {"c_F": 63, "c_node": 11, "c_vv": 48, "c_vf": 106},
 }
 
+# exact counts of the surface and update kernels per element (reference loop
+# structure: per face point two node conversions, one flux, the LF penalty and
+# the Nq x Nf lift; four Nq x Nq and one Nf x Nq mat-vec per field plus the
+# variable maps), keyed by (dim, Nq, Nf)
+SURF_UPDATE = {
+    (3, 23, 48): (19747, 41958),
+    (3, 44, 64): (39836, 134632),
+    (2, 12, 12): (3084, 8280),
+}
+
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 
@@ -50,6 +60,21 @@ def vol_flops_reference_structure(dim, Nq, Nf):
     p = PIECES[dim]
     nflux = Nq * Nq + 2 * Nq * Nf
     return nflux * (p["c_F"] + p["c_node"]) + Nq * Nq * p["c_vv"] + Nq * Nf * p["c_vf"]
+
+
+def stage_flops(dim, Nq, Nf):
+    """Volume (symmetric count) + surface + update flops of one element-stage."""
+    sf, uf = SURF_UPDATE[(dim, Nq, Nf)]
+    return vol_flops(dim, Nq, Nf) + sf + uf
+
+
+def bytes_stage_fused(dim, Nq, Nf):
+    """Compulsory traffic of the fused stage: state, traces (own + neighbour),
+    geometry, map and J in; Q, res, projected state and new traces out."""
+    nfld, ng = dim + 2, dim * dim
+    rd = nfld * Nq + 2 * nfld * Nf + ng * (Nq + Nf) + (dim + 1) * Nf + Nq + 2 * nfld * Nq
+    wr = 3 * nfld * Nq + nfld * Nf
+    return 8 * (rd + wr) + 4 * Nf
 
 
 def bytes_vol(dim, Nq, Nf):

@@ -47,6 +47,9 @@ def test_workmodel_matches_oracle_flop_counts(oracle):
         c = oracle.flop_counts(dim, Nq, Nfq)
         for k in ("c_F", "c_node", "c_vv", "c_vf"):
             assert W.PIECES[dim][k] == c[k], (dim, k)
+    for (dim, Nq, Nf), (sf, uf) in W.SURF_UPDATE.items():
+        c = oracle.flop_counts(dim, Nq, Nf // (dim + 1))
+        assert (sf, uf) == (c["surf_elem"], c["update_elem"]), (dim, Nq)
     assert W.vol_flops(3, 23, 48) == 290244
     assert W.bytes_vol(3, 23, 48) == 10792
     assert W.bytes_surf(3, 23, 48) == 9328

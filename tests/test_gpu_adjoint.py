@@ -1,0 +1,133 @@
+"""Discrete adjoint on the GPU: dot-product tests of each VJP kernel against
+central finite differences of the matching forward kernel, and the gradient of
+a density misfit through LSRK45 steps against finite differences."""
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import small_problem
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-6
+
+
+def _solver(prob):
+    from scratch_wadge_amd.device import DeviceSolver
+    return DeviceSolver(prob)
+
+
+def _dev(x):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device="cuda:0")
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("dim,N", [(2, 3), (3, 3), (2, 4)])
+def test_rhs_vjp_dot(dim, N):
+    from scratch_wadge_amd.device import _ptr
+    prob = small_problem(dim, N)
+    print(f"case rhs vjp dim={dim} N={N} K={prob.K}", flush=True)
+    dev = _solver(prob)
+    rng = np.random.default_rng(0)
+    uq, uf = prob.rhsQ.copy(), prob.Qf.copy()
+    Rb = rng.standard_normal(uq.shape)
+    dq = rng.standard_normal(uq.shape) * np.abs(uq)
+    df = rng.standard_normal(uf.shape) * np.abs(uf)
+    # adjoint side
+    c = dev.c
+    Rbd, sbar = _dev(Rb), torch.empty_like(dev.s["Qf"])
+    uqb, ufb = torch.empty_like(dev.s["rhsQ"]), torch.empty_like(dev.s["Qf"])
+    uqd, ufd = _dev(uq), _dev(uf)
+    dev.plan.rhs_vjp(prob.K, _ptr(c["vgeo"]), _ptr(c["vfgeo"]), _ptr(c["fgeo"]), _ptr(c["mapPq"]),
+                     _ptr(c["nrJ"]), _ptr(c["nsJ"]), _ptr(c["ntJ"]), _ptr(c["Drq"]), _ptr(c["Dsq"]),
+                     _ptr(c["Dtq"]), _ptr(c["VqLq"]), _ptr(c["VfPq"]), _ptr(uqd), _ptr(ufd),
+                     _ptr(Rbd), _ptr(sbar), _ptr(uqb), _ptr(ufb), dev.stream())
+    torch.cuda.synchronize()
+    adj = float(np.dot(uqb.cpu().numpy(), dq) + np.dot(ufb.cpu().numpy(), df))
+    rp, _ = dev.rhs(uq + EPS * dq, uf + EPS * df)
+    rm, _ = dev.rhs(uq - EPS * dq, uf - EPS * df)
+    fd = float(np.dot(Rb, (rp - rm) / (2 * EPS)))
+    print("rhs vjp: adjoint", adj, "fd", fd, "rel", _rel(adj, fd), flush=True)
+    assert _rel(adj, fd) < 1e-6
+
+
+@pytest.mark.parametrize("dim,N", [(2, 3), (3, 3)])
+def test_update_vjp_dot(dim, N):
+    from scratch_wadge_amd.device import _ptr
+    from scratch_wadge_amd.problem import RK4A, RK4B
+    prob = small_problem(dim, N)
+    print(f"case update vjp dim={dim} N={N} K={prob.K}", flush=True)
+    dev = _solver(prob)
+    rng = np.random.default_rng(1)
+    Q = prob.Q.copy()
+    R = 0.1 * rng.standard_normal(Q.shape)
+    res = 1e-3 * rng.standard_normal(Q.shape)
+    fa, fb, dt = RK4A[3], RK4B[3], 2e-3
+    # output cotangents
+    Qb_o = rng.standard_normal(Q.shape)
+    rb_o = rng.standard_normal(Q.shape)
+    uqb_o = rng.standard_normal(Q.shape)
+    ufb_o = rng.standard_normal(prob.Qf.shape)
+    # directions for the inputs (Q, res, R)
+    dQ = rng.standard_normal(Q.shape) * np.abs(Q) * 1e-3
+    dr = rng.standard_normal(Q.shape) * 1e-3
+    dR = rng.standard_normal(Q.shape) * 1e-1
+
+    def fwd(Qi, resi, Ri):
+        dev.set_state(Q=Qi, resQ=resi, rhsQ=Ri, Qf=prob.Qf)
+        dev.update(fa, fb, dt)
+        g = dev.get_state()
+        return g["Q"], g["resQ"], g["rhsQ"], g["Qf"]
+
+    Qo, ro, uqo, ufo = fwd(Q, res, R)
+    c = dev.c
+    Qb, rb = _dev(Qb_o), _dev(rb_o)
+    Rb = torch.empty_like(Qb)
+    dev.plan.update_vjp(prob.K, _ptr(c["Jq"]), _ptr(c["VqPq"]), _ptr(c["VfPq"]), fa, fb, dt,
+                        _ptr(_dev(Qo)), _ptr(Qb), _ptr(rb), _ptr(_dev(uqb_o)), _ptr(_dev(ufb_o)),
+                        _ptr(Rb), dev.stream())
+    torch.cuda.synchronize()
+    adj = float(np.dot(Qb.cpu().numpy(), dQ) + np.dot(rb.cpu().numpy(), dr) + np.dot(Rb.cpu().numpy(), dR))
+    p = fwd(Q + EPS * dQ, res + EPS * dr, R + EPS * dR)
+    m = fwd(Q - EPS * dQ, res - EPS * dr, R - EPS * dR)
+    cots = (Qb_o, rb_o, uqb_o, ufb_o)
+    fd = float(sum(np.dot(cb, (a - b) / (2 * EPS)) for cb, a, b in zip(cots, p, m)))
+    print("update vjp: adjoint", adj, "fd", fd, "rel", _rel(adj, fd), flush=True)
+    assert _rel(adj, fd) < 1e-6
+
+
+@pytest.mark.parametrize("dim,N,steps,ckpt", [(2, 3, 6, 4), (3, 3, 3, 2)])
+def test_trajectory_gradient_fd(dim, N, steps, ckpt):
+    """Gradient of the density misfit after several LSRK45 steps with respect to
+    the initial conservative state, against central differences on seeded
+    directions."""
+    from scratch_wadge_amd.adjoint import AdjointSolver, RKRollout, density_misfit
+    prob = small_problem(dim, N)
+    print(f"case trajectory dim={dim} N={N} K={prob.K} steps={steps}", flush=True)
+    dev = _solver(prob)
+    adj = AdjointSolver(dev, checkpoint_every=ckpt)
+    rng = np.random.default_rng(3)
+    Q0 = _dev(prob.Q)
+    target = _dev(prob.Q.reshape(prob.K, prob.nfields, prob.Nq)[:, 0, :].reshape(-1)
+                  * (1 + 1e-2 * rng.standard_normal(prob.K * prob.Nq)))
+    q = Q0.clone().requires_grad_(True)
+    L = density_misfit(prob, RKRollout.apply(q, adj, steps), target)
+    L.backward()
+    g = q.grad.cpu().numpy()
+
+    def loss(Qi):
+        with torch.no_grad():
+            return float(density_misfit(prob, adj.forward(Qi, steps), target))
+
+    worst = 0.0
+    for k in range(4):
+        d = rng.standard_normal(prob.Q.shape) * np.abs(prob.Q)
+        eps = 1e-6
+        fd = (loss(Q0 + eps * _dev(d)) - loss(Q0 - eps * _dev(d))) / (2 * eps)
+        worst = max(worst, _rel(float(g @ d), fd))
+        print("trajectory: adjoint", float(g @ d), "fd", fd, flush=True)
+    assert worst < 1e-6
