@@ -17,7 +17,15 @@
  * p_Nfields, p_gamma, p_TAU, p_Kblk*; euler.c:2880-2948) become a plan.  Every
  * pointer is a device pointer, every call is asynchronous on the given HIP
  * stream (a hipStream_t passed as void*, NULL = default stream), no call
- * allocates or synchronises, and the calls can be captured in a HIP graph.
+ * allocates, and the calls can be captured in a HIP graph.
+ *
+ * The volume-type calls (wadges_vol, wadges_stage, wadges_rhs_vjp) read the
+ * operators through tables the plan re-lays out on the device.  The tables are
+ * prepared on the first such call, and again whenever the operator pointers
+ * (Drq, Dsq, Dtq, VqLq, VfPq) differ from those of the previous preparation;
+ * that call waits for the preparation (outside stream capture; under capture
+ * the preparation is recorded into the graph).  After changing operator values
+ * in place, call wadges_plan_invalidate.
  *
  * Layouts (Appendix of the reference's operators_c.cpp:322-389 and
  * euler.c:3158-3170), all FP64 except mapPq:
@@ -55,10 +63,14 @@ typedef struct wadges_plan wadges_plan;
  * degree; Nq, Nfq the volume / per-face quadrature sizes the operators were
  * built with; gamma = p_gamma; tau = p_TAU (the reference's 2D surface kernel
  * has no tau and equals tau = 1).  Allocates the plan's small device workspace
- * on the current device; this is the only call that allocates. */
+ * on the current device; of the single-problem calls this is the only one that
+ * allocates (the ensemble calls wadges_batch_create, _set_config and _tune
+ * allocate too). */
 int wadges_plan_create(wadges_plan **plan, int dim, int N, int Nq, int Nfq, double gamma,
                        double tau);
 int wadges_plan_destroy(wadges_plan *plan);
+/* Forgets the prepared operator tables (see above); the next call prepares them. */
+int wadges_plan_invalidate(wadges_plan *plan);
 
 /* Volume kernel: role of euler_vol_3d_curved (okl/Euler3D.okl:2123-2523) and
  * euler_vol_2d_curved (okl/Euler2D.okl:773-1353).  Reads the entropy-projected
@@ -148,8 +160,74 @@ int wadges_diagnostics(const wadges_plan *plan, int K, const double *wq, const d
                        const double *Q, const double *Qexact, double *aux, double *totals,
                        void *stream);
 
+/* ---- Ensembles: many independent meshes (uneven K and N, one dim) advanced
+ * together, one launch per LSRK stage for the whole ensemble.  No counterpart
+ * in the reference, which runs one mesh per process through the same stage
+ * sequence (rk_run, euler.c:3672-3699); per problem the arithmetic is exactly
+ * that of wadges_stage.
+ *
+ * Each problem is described by device pointers in the layouts above.  Its face
+ * traces are double-buffered: Qf[0] holds the current traces at create, and
+ * wadges_batch_current tells which of Qf[0], Qf[1] holds them after any number
+ * of stages.  Operator tables (Drq .. VfPq, nrJ ..) are re-laid out once at
+ * create: they must not change afterwards.  Problems with equal N may share
+ * operator arrays. */
+typedef struct wadges_group {
+  int N, Nq, Nfq;   /* kernel instance, as for wadges_plan_create */
+  int K;            /* elements (0 allowed) */
+  double dt;        /* this problem's time step */
+  const double *vgeo, *vfgeo, *fgeo;
+  const uint32_t *mapPq;
+  const double *nrJ, *nsJ, *ntJ, *Drq, *Dsq, *Dtq, *VqLq, *VqPq, *VfPq, *Jq;
+  double *Q, *rhsQ, *resQ;
+  double *Qf[2];
+} wadges_group;
+
+typedef struct wadges_batch wadges_batch;
+
+/* Launch shape of the ensemble kernel.  A tile is the floor(64/Nq) elements
+ * one 64-wide workgroup holds.  grid = 0: one workgroup per tile (no loop).
+ * grid > 0: a persistent launch of grid workgroups over items of
+ * tiles_per_item tiles, taken in a static stride (dynamic = 0) or from an
+ * atomic work counter (dynamic = 1).  order = 1 issues the costliest items
+ * first, 0 keeps problem order. */
+typedef struct wadges_batch_config {
+  int tiles_per_item;
+  int grid;
+  int dynamic;
+  int order;
+} wadges_batch_config;
+
+/* Copies the descriptors, checks them (instances exist, pointers set, Qf[0] !=
+ * Qf[1]), allocates the device tables and prepares the operator tables on
+ * stream.  The default launch shape is used until wadges_batch_tune or
+ * wadges_batch_set_config. */
+int wadges_batch_create(wadges_batch **batch, int dim, int ngroups, const wadges_group *groups,
+                        double gamma, double tau, void *stream);
+int wadges_batch_destroy(wadges_batch *batch);
+
+/* One stage fa, fb for every problem (fdt = its dt); flips the current trace. */
+int wadges_batch_stage(wadges_batch *batch, double fa, double fb, void *stream);
+
+/* nsteps LSRK45 steps (five stages each, coefficients of euler.c:2779-2790).
+ * use_graph = 1 replays a HIP graph of one step (captured at the first call and
+ * after any change of launch shape); 0 launches the stages directly. */
+int wadges_batch_run(wadges_batch *batch, int nsteps, int use_graph, void *stream);
+
+int wadges_batch_current(const wadges_batch *batch);   /* 0 or 1: index into Qf[] */
+int wadges_batch_get_config(const wadges_batch *batch, wadges_batch_config *cfg);
+int wadges_batch_set_config(wadges_batch *batch, const wadges_batch_config *cfg);
+
+/* Autotuner: times every candidate launch shape over reps stages each (HIP
+ * events on stream, synchronising), keeps the fastest and returns its mean
+ * stage time in *ms (may be NULL).  The state of every problem is saved first
+ * and restored afterwards.  Results are cached per ensemble signature (dim and
+ * the sorted (Nq, K) list) for the process, and in the text file named by the
+ * environment variable WADGES_TUNE_CACHE when it is set. */
+int wadges_batch_tune(wadges_batch *batch, int reps, double *ms, void *stream);
+
 /* Introspection. */
-int wadges_last_hip_error(void);           /* hipError_t of the last failing HIP call */
+int wadges_last_hip_error(void);          /* hipError_t of the last failing HIP call */
 const char *wadges_error_string(int code);  /* text for a WADGES_E* code */
 int wadges_instance_count(void);            /* number of compiled (dim, Nq, Nfq) instances */
 int wadges_instance(int idx, int *dim, int *N, int *Nq, int *Nfq);

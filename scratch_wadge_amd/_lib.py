@@ -15,9 +15,11 @@ LIB_PATH = os.path.join(HERE, "libwadges.so")
 
 # Every symbol the header declares (checked by the CPU tests).
 EXPORTED = (
-    "wadges_plan_create", "wadges_plan_destroy", "wadges_vol", "wadges_surf", "wadges_update",
+    "wadges_plan_create", "wadges_plan_destroy", "wadges_plan_invalidate", "wadges_vol", "wadges_surf", "wadges_update",
     "wadges_stage", "wadges_diagnostics", "wadges_rhs_vjp", "wadges_update_vjp",
-    "wadges_project_vjp",
+    "wadges_project_vjp", "wadges_batch_create", "wadges_batch_destroy", "wadges_batch_stage",
+    "wadges_batch_run", "wadges_batch_current", "wadges_batch_get_config", "wadges_batch_set_config",
+    "wadges_batch_tune",
     "wadges_last_hip_error", "wadges_error_string", "wadges_instance_count", "wadges_instance",
 )
 
@@ -27,6 +29,24 @@ _lib = None
 
 class WadgesError(RuntimeError):
     pass
+
+
+class Group(ctypes.Structure):
+    """struct wadges_group: one problem of an ensemble (device pointers)."""
+    _fields_ = [("N", ctypes.c_int), ("Nq", ctypes.c_int), ("Nfq", ctypes.c_int), ("K", ctypes.c_int),
+                ("dt", ctypes.c_double)] + [
+        (name, _vp) for name in ("vgeo", "vfgeo", "fgeo", "mapPq", "nrJ", "nsJ", "ntJ", "Drq", "Dsq",
+                                 "Dtq", "VqLq", "VqPq", "VfPq", "Jq", "Q", "rhsQ", "resQ")] + [
+        ("Qf", _vp * 2)]
+
+
+class BatchConfig(ctypes.Structure):
+    """struct wadges_batch_config: launch shape of the ensemble kernel."""
+    _fields_ = [("tiles_per_item", ctypes.c_int), ("grid", ctypes.c_int), ("dynamic", ctypes.c_int),
+                ("order", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def load():
@@ -39,6 +59,7 @@ def load():
     L.wadges_plan_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.wadges_plan_destroy.argtypes = [_vp]
+    L.wadges_plan_invalidate.argtypes = [_vp]
     L.wadges_vol.argtypes = [_vp, ctypes.c_int] + [_vp] * 15 + [_vp]
     L.wadges_surf.argtypes = [_vp, ctypes.c_int] + [_vp] * 6 + [_vp]
     L.wadges_update.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
@@ -49,6 +70,15 @@ def load():
     L.wadges_update_vjp.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double] + [_vp] * 6 + [_vp]
     L.wadges_project_vjp.argtypes = [_vp, ctypes.c_int] + [_vp] * 6 + [_vp]
+    L.wadges_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(Group), ctypes.c_double, ctypes.c_double, _vp]
+    L.wadges_batch_destroy.argtypes = [_vp]
+    L.wadges_batch_stage.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _vp]
+    L.wadges_batch_run.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp]
+    L.wadges_batch_current.argtypes = [_vp]
+    L.wadges_batch_get_config.argtypes = [_vp, ctypes.POINTER(BatchConfig)]
+    L.wadges_batch_set_config.argtypes = [_vp, ctypes.POINTER(BatchConfig)]
+    L.wadges_batch_tune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _vp]
     L.wadges_error_string.restype = ctypes.c_char_p
     L.wadges_error_string.argtypes = [ctypes.c_int]
     L.wadges_instance.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4
@@ -94,6 +124,9 @@ class Plan:
         except Exception:
             pass
 
+    def invalidate(self):
+        check(load().wadges_plan_invalidate(self.h), "wadges_plan_invalidate")
+
     def vol(self, K, vgeo, vfgeo, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, Drstq, VqLq, VfPq, Q, Qf, rhsQ,
             rhsQf, stream=None):
         check(load().wadges_vol(self.h, K, vgeo, vfgeo, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, Drstq, VqLq,
@@ -130,3 +163,51 @@ class Plan:
     def update(self, K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ, Q, Qf, stream=None):
         check(load().wadges_update(self.h, K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ, Q, Qf, stream),
               "wadges_update")
+
+
+class Batch:
+    """Owns a wadges_batch: an ensemble of problems advanced by one launch per
+    stage.  `groups` is a sequence of Group structures; the caller keeps the
+    device arrays they point to alive for the life of the batch."""
+
+    def __init__(self, dim, groups, gamma, tau, stream=None):
+        L = load()
+        arr = (Group * len(groups))(*groups)
+        h = _vp()
+        check(L.wadges_batch_create(ctypes.byref(h), dim, len(groups), arr, gamma, tau, stream),
+              "wadges_batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().wadges_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stage(self, fa, fb, stream=None):
+        check(load().wadges_batch_stage(self.h, fa, fb, stream), "wadges_batch_stage")
+
+    def run(self, nsteps, use_graph=True, stream=None):
+        check(load().wadges_batch_run(self.h, nsteps, 1 if use_graph else 0, stream), "wadges_batch_run")
+
+    def current(self):
+        return load().wadges_batch_current(self.h)
+
+    def config(self):
+        c = BatchConfig()
+        check(load().wadges_batch_get_config(self.h, ctypes.byref(c)), "wadges_batch_get_config")
+        return c.as_dict()
+
+    def set_config(self, tiles_per_item=1, grid=0, dynamic=0, order=0):
+        c = BatchConfig(tiles_per_item, grid, dynamic, order)
+        check(load().wadges_batch_set_config(self.h, ctypes.byref(c)), "wadges_batch_set_config")
+
+    def tune(self, reps=5, stream=None):
+        ms = ctypes.c_double()
+        check(load().wadges_batch_tune(self.h, reps, ctypes.byref(ms), stream), "wadges_batch_tune")
+        return ms.value

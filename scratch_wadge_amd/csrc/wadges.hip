@@ -27,11 +27,18 @@
 // addresses that all teams of all waves share (L1 hits).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
 #include <new>
+#include <string>
+#include <vector>
 
 #include "wadges_device.hpp"
 #include "../../include/wadges.h"
@@ -531,45 +538,63 @@ update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ V
 // chip.  Neighbours read face traces of the previous stage while this element
 // writes its new ones, so the face traces are double-buffered: Qf_in is read,
 // Qf_out written (the caller swaps them between stages).
+
+// Arguments of one fused stage over one mesh; the ensemble kernel keeps one per
+// problem in device memory.
+struct StageArgs {
+  int K;
+  const double *vgeo, *vfgeo, *fgeo;
+  const uint32_t *mapPq;
+  const double *nrJ, *nsJ, *ntJ, *ws, *VqLq, *VqPq, *VfPq, *Jq;
+  double fa, fb, fdt;
+  const double *Qf_in;
+  double *Qf_out, *rhsQ, *resQ, *Q;
+  double gamma, gm1, inv2gm1, ltau, lgm1;
+};
+
+// Stage arguments are read through the constant address space, where they
+// live (kernel arguments, or descriptor tables written by the host): loads are
+// scalar, can be repeated at each use instead of held in registers, and the
+// pointers loaded from there are known to point to global memory.
+typedef __attribute__((address_space(4))) const StageArgs ConstArgs;
+
+// One fused stage for the TPW elements kfirst .. kfirst+TPW-1 (one team of NQ
+// lanes each) of the mesh described by A.  All lanes of the workgroup call it.
+// fa, fb and the face-trace buffers come separately from A so that an ensemble
+// can keep A in device memory and vary them per launch.  lane is threadIdx.x
+// (passed in so that a caller looping over tiles can hand an opaque copy).
 template <class S, int TPW>
-__global__ void __launch_bounds__(WAVE)
-stage_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vfgeo,
-             const double *__restrict__ fgeo, const uint32_t *__restrict__ mapPq,
-             const double *__restrict__ nrJ, const double *__restrict__ nsJ,
-             const double *__restrict__ ntJ, const double *__restrict__ ws,
-             const double *__restrict__ VqLq, const double *__restrict__ VqPq,
-             const double *__restrict__ VfPq, const double *__restrict__ Jq, double fa, double fb,
-             double fdt, const double *__restrict__ Qf_in, double *__restrict__ Qf_out,
-             double *rhsQ, double *__restrict__ resQ, double *__restrict__ Q, double gamma,
-             double gm1, double inv2gm1, double ltau, double lgm1) {
+__device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double fb,
+                                           const double *__restrict__ Qf_in,
+                                           double *__restrict__ Qf_out, int kfirst, int lane,
+                                           double (*s_nref)[S::DIM], double *smem) {
   constexpr int DIM = S::DIM, NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD, NG = S::NG;
   constexpr int FREC = S::FREC, NFG = DIM + 1;
   constexpr int PER = VolSmem<S>::per_elem;
-  extern __shared__ double smem[];
-  __shared__ double s_nref[S::NFACES][DIM];
+  const double gamma = A->gamma, gm1 = A->gm1, inv2gm1 = A->inv2gm1, ltau = A->ltau, lgm1 = A->lgm1;
 
-  const int lane = threadIdx.x;
   const int team = lane / NQ;
   const int i = lane - team * NQ;
   const bool active = team < TPW;
-  const int k = blockIdx.x * TPW + team;
-  const bool valid = active && k < K;
+  const int k = kfirst + team;
+  const bool valid = active && k < A->K;
   const size_t kk = valid ? (size_t)k : 0;
   const size_t base = kk * NQ * NFLD;
+  const double fdt = A->fdt;
 
-  load_nref<S>(lane, nrJ, nsJ, ntJ, s_nref);
+  load_nref<S>(lane, A->nrJ, A->nsJ, A->ntJ, s_nref);
   double *accV = smem + (active ? team : 0) * PER;
   double *accF = accV + NFLD * NQ + FREC * NFN;
   double acc[NFLD];
-  vol_body<S>(valid, i, rhsQ + base, Qf_in + kk * NFN * NFLD, vgeo + kk * NQ * NG,
-              vfgeo + kk * NFN * NG, s_nref, ws, accV, gm1, inv2gm1, acc);
+  vol_body<S>(valid, i, A->rhsQ + base, Qf_in + kk * NFN * NFLD, A->vgeo + kk * NQ * NG,
+              A->vfgeo + kk * NFN * NG, s_nref, A->ws, accV, gm1, inv2gm1, acc);
 
   // ---------------- surface flux + LF penalty, in place over the face rows
   if (valid) {
-    const double *g = fgeo + kk * NFN * NFG;
+    const double *g = A->fgeo + kk * NFN * NFG;
     for (int fn = i; fn < NFN; fn += NQ) {
       const size_t idM = (size_t)fn + kk * NFN * NFLD;
-      const size_t idP = mapPq[kk * NFN + fn];
+      const size_t idP = A->mapPq[kk * NFN + fn];
       double UM[NFLD], UP[NFLD];
       for (int f = 0; f < NFLD; ++f) {
         UM[f] = Qf_in[idM + (size_t)f * NFN];
@@ -600,7 +625,7 @@ stage_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ 
     for (int f = 0; f < NFLD; ++f) val[f] = 0.0;
 #pragma unroll 4
     for (int j = 0; j < NFN; ++j) {
-      const double L = VqLq[i + j * NQ];
+      const double L = A->VqLq[i + j * NQ];
       for (int f = 0; f < NFLD; ++f) val[f] += L * accF[f * NFN + j];
     }
     for (int f = 0; f < NFLD; ++f) rhs[f] = (acc[f] + accV[f * NQ + i]) + val[f];
@@ -612,7 +637,7 @@ stage_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ 
   double *bufB = accV + NFLD * NQ;    // [NFLD][NQ]
   double J = 1.0, invJ = 1.0;
   if (valid) {
-    J = Jq[kk * NQ + i];
+    J = A->Jq[kk * NQ + i];
     invJ = 1.0 / J;
     for (int f = 0; f < NFLD; ++f) bufA[f * NQ + i] = rhs[f];
   }
@@ -621,7 +646,7 @@ stage_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ 
     for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
 #pragma unroll 2
     for (int j = 0; j < NQ; ++j) {
-      const double P = scale * VqPq[i + j * NQ];
+      const double P = scale * A->VqPq[i + j * NQ];
       for (int f = 0; f < NFLD; ++f) r[f] += P * in[f * NQ + j];
     }
   };
@@ -636,10 +661,10 @@ stage_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ 
     matvec(bufB, 1.0, r);
     for (int f = 0; f < NFLD; ++f) {
       const size_t id = base + i + f * NQ;
-      const double res = fa * resQ[id] + fdt * r[f];
-      resQ[id] = res;
-      U[f] = Q[id] + fb * res;
-      Q[id] = U[f];
+      const double res = fa * A->resQ[id] + fdt * r[f];
+      A->resQ[id] = res;
+      U[f] = A->Q[id] + fb * res;
+      A->Q[id] = U[f];
     }
     entropy_vars<DIM>(U, gamma, V);
     for (int f = 0; f < NFLD; ++f) bufA[f * NQ + i] = V[f] * J;
@@ -664,7 +689,7 @@ stage_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ 
       for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
 #pragma unroll 2
       for (int j = 0; j < NQ; ++j) {
-        const double P = VfPq[fn + j * NFN];
+        const double P = A->VfPq[fn + j * NFN];
         for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[f * NQ + j];
       }
       conservative_vars<DIM>(V, gamma, lgm1, U);
@@ -673,7 +698,7 @@ stage_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ 
     double V[NFLD], U[NFLD];
     matvec(bufA, 1.0, V);
     conservative_vars<DIM>(V, gamma, lgm1, U);
-    for (int f = 0; f < NFLD; ++f) rhsQ[base + i + f * NQ] = U[f];
+    for (int f = 0; f < NFLD; ++f) A->rhsQ[base + i + f * NQ] = U[f];
   }
 }
 
@@ -745,6 +770,15 @@ diag_reduce_kernel(int K, const double *__restrict__ aux, double *__restrict__ t
   if (t < NDIAG) totals[t] = red[t][0];
 }
 
+
+template <class S, int TPW>
+__global__ void __launch_bounds__(WAVE) stage_kernel(StageArgs A) {
+  extern __shared__ double smem[];
+  __shared__ double s_nref[S::NFACES][S::DIM];
+  stage_body<S, TPW>((const ConstArgs *)&A, A.fa, A.fb, A.Qf_in, A.Qf_out, blockIdx.x * TPW,
+                     threadIdx.x, s_nref, smem);
+}
+
 #include "wadges_adjoint.hpp"
 
 // ------------------------------------------------------------------ dispatch
@@ -783,7 +817,10 @@ struct Ops {
                     const double *, double *, hipStream_t);
   int (*proj_vjp)(const wadges_plan *, int, const double *, const double *, const double *,
                   const double *, const double *, double *, hipStream_t);
+  int (*prep)(const double *, const double *, const double *, const double *, const double *,
+              double *, hipStream_t);
   int ws_doubles;
+  int stage_smem;  // dynamic LDS bytes of one fused-stage workgroup
 };
 
 }  // namespace wadges
@@ -794,9 +831,33 @@ struct wadges_plan {
   double *ws;  // device workspace (operator tables)
   int device;
   wadges::Ops ops;
+  // Operator pointers the tables in ws were prepared from (null: not prepared).
+  mutable const double *tab_src[5];
 };
 
 namespace wadges {
+
+// Prepares the operator tables of the plan unless they were prepared from the
+// same operator pointers before.  Outside stream capture a fresh preparation
+// is waited for, so the tables are then valid for calls on every stream; under
+// capture a needed preparation is recorded into the graph and the cache is
+// left alone (the tables become valid only when the graph runs).
+template <class S>
+int ensure_tables(const wadges_plan *p, const double *Drq, const double *Dsq, const double *Dtq,
+                  const double *VqLq, const double *VfPq, hipStream_t st) {
+  const double *src[5] = {Drq, Dsq, Dtq, VqLq, VfPq};
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  int rc = hip_check(hipStreamIsCapturing(st, &cs));
+  if (rc) return rc;
+  const bool capturing = cs != hipStreamCaptureStatusNone;
+  if (std::equal(src, src + 5, p->tab_src)) return WADGES_OK;
+  prep_vol_tables<S><<<(S::WS + 255) / 256, 256, 0, st>>>(Drq, Dsq, Dtq, VqLq, VfPq, p->ws);
+  if ((rc = hip_check(hipGetLastError()))) return rc;
+  if (capturing) return WADGES_OK;
+  if ((rc = hip_check(hipStreamSynchronize(st)))) return rc;
+  std::copy(src, src + 5, p->tab_src);
+  return WADGES_OK;
+}
 
 template <class S> int vol_impl(const wadges_plan *p, int K, const double *vgeo,
                                 const double *vfgeo, const double *nrJ, const double *nsJ,
@@ -805,9 +866,7 @@ template <class S> int vol_impl(const wadges_plan *p, int K, const double *vgeo,
                                 const double *Qf, double *rhsQ, double *rhsQf, hipStream_t st) {
   if (K == 0) return WADGES_OK;
   {
-    const int n = S::WS;
-    prep_vol_tables<S><<<(n + 255) / 256, 256, 0, st>>>(Drq, Dsq, Dtq, VqLq, VfPq, p->ws);
-    int rc = hip_check(hipGetLastError());
+    int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
     if (rc) return rc;
   }
   constexpr int TPW = WAVE / S::NQ;
@@ -858,16 +917,16 @@ template <class S> int stage_impl(const wadges_plan *p, int K, const double *vge
                                   const double *Qf_in, double *Qf_out, double *rhsQ, double *resQ,
                                   double *Q, hipStream_t st) {
   if (K == 0) return WADGES_OK;
-  prep_vol_tables<S><<<(S::WS + 255) / 256, 256, 0, st>>>(Drq, Dsq, Dtq, VqLq, VfPq, p->ws);
-  int rc = hip_check(hipGetLastError());
+  int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
   if (rc) return rc;
   constexpr int TPW = WAVE / S::NQ;
   const size_t smem = (size_t)TPW * VolSmem<S>::per_elem * sizeof(double);
   const int blocks = (K + TPW - 1) / TPW;
   const double g = p->gamma;
-  stage_kernel<S, TPW><<<blocks, WAVE, smem, st>>>(
-      K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, p->ws, VqLq, VqPq, VfPq, Jq, fa, fb, fdt, Qf_in,
-      Qf_out, rhsQ, resQ, Q, g, g - 1.0, 1.0 / (2.0 * (g - 1.0)), 0.25 * p->tau, std::log(g - 1.0));
+  StageArgs A{K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, p->ws, VqLq, VqPq, VfPq, Jq, fa, fb,
+              fdt, Qf_in, Qf_out, rhsQ, resQ, Q, g, g - 1.0, 1.0 / (2.0 * (g - 1.0)),
+              0.25 * p->tau, std::log(g - 1.0)};
+  stage_kernel<S, TPW><<<blocks, WAVE, smem, st>>>(A);
   return hip_check(hipGetLastError());
 }
 
@@ -892,8 +951,7 @@ template <class S> int rhs_vjp_impl(const wadges_plan *p, int K, const double *v
                                     const double *Uf, const double *Rb, double *sbar, double *uqb,
                                     double *ufb, hipStream_t st) {
   if (K == 0) return WADGES_OK;
-  prep_vol_tables<S><<<(S::WS + 255) / 256, 256, 0, st>>>(Drq, Dsq, Dtq, VqLq, VfPq, p->ws);
-  int rc = hip_check(hipGetLastError());
+  int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
   if (rc) return rc;
   {
     constexpr int TS = Max<S::NQ, S::NFN>::value;
@@ -936,6 +994,17 @@ template <class S> int proj_vjp_impl(const wadges_plan *p, int K, const double *
   return hip_check(hipGetLastError());
 }
 
+template <class S> int prep_impl(const double *Drq, const double *Dsq, const double *Dtq,
+                                 const double *VqLq, const double *VfPq, double *ws,
+                                 hipStream_t st) {
+  prep_vol_tables<S><<<(S::WS + 255) / 256, 256, 0, st>>>(Drq, Dsq, Dtq, VqLq, VfPq, ws);
+  return hip_check(hipGetLastError());
+}
+
+template <class S> constexpr int stage_smem_bytes() {
+  return (WAVE / S::NQ) * VolSmem<S>::per_elem * (int)sizeof(double);
+}
+
 struct Instance {
   int dim, N, Nq, Nfq;
   Ops ops;
@@ -945,7 +1014,8 @@ template <int DIM, int NQ, int NFQ> constexpr Instance make_instance(int N) {
   using S = Shape<DIM, NQ, NFQ>;
   return Instance{DIM, N, NQ, NFQ,
                   Ops{&vol_impl<S>, &surf_impl<S>, &update_impl<S>, &stage_impl<S>, &diag_impl<S>,
-                      &rhs_vjp_impl<S>, &update_vjp_impl<S>, &proj_vjp_impl<S>, S::WS}};
+                      &rhs_vjp_impl<S>, &update_vjp_impl<S>, &proj_vjp_impl<S>, &prep_impl<S>,
+                      S::WS, stage_smem_bytes<S>()}};
 }
 
 // (dim, N, Nq, Nfq): quadrature of degree 2N, the point counts of the reference's
@@ -958,6 +1028,8 @@ const Instance kInstances[] = {
 constexpr int kNumInstances = sizeof(kInstances) / sizeof(kInstances[0]);
 
 }  // namespace wadges
+
+#include "wadges_batch.hpp"
 
 // ================================================================== C ABI
 
@@ -1013,6 +1085,7 @@ int wadges_plan_create(wadges_plan **plan, int dim, int N, int Nq, int Nfq, doub
   p->tau = tau;
   p->ops = inst->ops;
   p->ws = nullptr;
+  for (auto &t : p->tab_src) t = nullptr;
   if (hipGetDevice(&p->device) != hipSuccess) {
     delete p;
     return WADGES_EHIP;
@@ -1033,6 +1106,12 @@ int wadges_plan_destroy(wadges_plan *plan) {
   if (plan->ws) rc = hip_check(hipFree(plan->ws));
   delete plan;
   return rc;
+}
+
+int wadges_plan_invalidate(wadges_plan *plan) {
+  if (!plan) return WADGES_EINVAL;
+  for (auto &t : plan->tab_src) t = nullptr;
+  return WADGES_OK;
 }
 
 int wadges_vol(const wadges_plan *p, int K, const double *vgeo, const double *vfgeo,

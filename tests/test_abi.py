@@ -54,3 +54,39 @@ def test_workmodel_matches_oracle_flop_counts(oracle):
     assert W.bytes_vol(3, 23, 48) == 10792
     assert W.bytes_surf(3, 23, 48) == 9328
     assert W.bytes_update(3, 23, 48) == 7624
+
+
+def test_batch_argument_checks():
+    """wadges_batch_create rejects bad descriptors before any HIP call."""
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    fake = 0x1000      # never dereferenced: validation fails first
+
+    def group(Nq=12, Nfq=4, K=8, same_qf=False, dt=1e-3):
+        g = _lib.Group()
+        g.N, g.Nq, g.Nfq, g.K, g.dt = 3, Nq, Nfq, K, dt
+        for name in ("vgeo", "vfgeo", "fgeo", "mapPq", "nrJ", "nsJ", "Drq", "Dsq", "VqLq", "VqPq",
+                     "VfPq", "Jq", "Q", "rhsQ", "resQ"):
+            setattr(g, name, fake)
+        g.Qf[0] = fake
+        g.Qf[1] = fake if same_qf else fake + 8
+        return g
+
+    def create(dim, groups, gamma=1.4, tau=1.0):
+        arr = (_lib.Group * max(1, len(groups)))(*groups)
+        return L.wadges_batch_create(ctypes.byref(h), dim, len(groups), arr, gamma, tau, None)
+
+    assert create(4, [group()]) == -1                 # no such dimension
+    assert create(2, [group(Nq=13)]) == -2            # no kernel instance
+    assert create(2, [group(K=-1)]) == -1
+    assert create(2, [group(same_qf=True)]) == -1     # the stage reads one buffer, writes the other
+    assert create(2, [group(dt=float("nan"))]) == -1
+    assert create(3, [group(Nq=23, Nfq=12)]) == -1    # 3D needs ntJ and Dtq
+    assert create(2, [group()], gamma=1.0) == -1
+    assert L.wadges_batch_create(ctypes.byref(h), 2, -1, None, 1.4, 1.0, None) == -1
+    assert L.wadges_batch_current(None) == -1
+    assert L.wadges_batch_stage(None, 0.0, 0.0, None) == -1
+    assert L.wadges_batch_run(None, 1, 1, None) == -1
+    assert L.wadges_batch_tune(None, 1, None, None) == -1
+    assert L.wadges_plan_invalidate(None) == -1
+    assert L.wadges_batch_destroy(None) == 0

@@ -33,7 +33,11 @@ def test_rhs_vjp_dot(dim, N):
     print(f"case rhs vjp dim={dim} N={N} K={prob.K}", flush=True)
     dev = _solver(prob)
     rng = np.random.default_rng(0)
-    uq, uf = prob.rhsQ.copy(), prob.Qf.copy()
+    # Independent 1% noise on volume and face states: traces jump across every
+    # face, so max(lamM, lamP) of the penalty is far from its kink at the
+    # finite-difference step (a smooth state has lamM ~ lamP there).
+    uq = prob.rhsQ * (1.0 + 1e-2 * rng.standard_normal(prob.rhsQ.shape))
+    uf = prob.Qf * (1.0 + 1e-2 * rng.standard_normal(prob.Qf.shape))
     Rb = rng.standard_normal(uq.shape)
     dq = rng.standard_normal(uq.shape) * np.abs(uq)
     df = rng.standard_normal(uf.shape) * np.abs(uf)
@@ -87,9 +91,9 @@ def test_update_vjp_dot(dim, N):
     c = dev.c
     Qb, rb = _dev(Qb_o), _dev(rb_o)
     Rb = torch.empty_like(Qb)
+    Qod, uqbd, ufbd = _dev(Qo), _dev(uqb_o), _dev(ufb_o)    # held until the kernel has run
     dev.plan.update_vjp(prob.K, _ptr(c["Jq"]), _ptr(c["VqPq"]), _ptr(c["VfPq"]), fa, fb, dt,
-                        _ptr(_dev(Qo)), _ptr(Qb), _ptr(rb), _ptr(_dev(uqb_o)), _ptr(_dev(ufb_o)),
-                        _ptr(Rb), dev.stream())
+                        _ptr(Qod), _ptr(Qb), _ptr(rb), _ptr(uqbd), _ptr(ufbd), _ptr(Rb), dev.stream())
     torch.cuda.synchronize()
     adj = float(np.dot(Qb.cpu().numpy(), dQ) + np.dot(rb.cpu().numpy(), dr) + np.dot(Rb.cpu().numpy(), dR))
     p = fwd(Q + EPS * dQ, res + EPS * dr, R + EPS * dR)
@@ -111,7 +115,10 @@ def test_trajectory_gradient_fd(dim, N, steps, ckpt):
     dev = _solver(prob)
     adj = AdjointSolver(dev, checkpoint_every=ckpt)
     rng = np.random.default_rng(3)
-    Q0 = _dev(prob.Q)
+    # Per-node noise keeps the traces discontinuous (see test_rhs_vjp_dot), so
+    # that the finite differences, with a step well below the jumps, do not
+    # straddle the kink of max(lamM, lamP).
+    Q0 = _dev(prob.Q * (1.0 + 1e-2 * rng.standard_normal(prob.Q.shape)))
     target = _dev(prob.Q.reshape(prob.K, prob.nfields, prob.Nq)[:, 0, :].reshape(-1)
                   * (1 + 1e-2 * rng.standard_normal(prob.K * prob.Nq)))
     q = Q0.clone().requires_grad_(True)
@@ -126,7 +133,7 @@ def test_trajectory_gradient_fd(dim, N, steps, ckpt):
     worst = 0.0
     for k in range(4):
         d = rng.standard_normal(prob.Q.shape) * np.abs(prob.Q)
-        eps = 1e-6
+        eps = 1e-7
         fd = (loss(Q0 + eps * _dev(d)) - loss(Q0 - eps * _dev(d))) / (2 * eps)
         worst = max(worst, _rel(float(g @ d), fd))
         print("trajectory: adjoint", float(g @ d), "fd", fd, flush=True)
