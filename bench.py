@@ -38,6 +38,11 @@ def parse():
     ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--extras", dest="extras", action="store_true", default=True,
+                    help="also measure the ensemble and adjoint workloads (default)")
+    ap.add_argument("--no-extras", dest="extras", action="store_false")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="instead of the benchmark, copy a GIB-sized device buffer 5 times "
                          "(reference kernel of known traffic for the HBM counters)")
@@ -97,6 +102,76 @@ def cpu_baseline(prob, target_s):
     return {"value": gdof, "unit": "GDOF*stage/s", "cores": threads, "kind": "port",
             "sample": f"one vol+surf+update stage over {Ks} of {prob.K} elements ({dt:.1f} s, "
                       f"C++ oracle, OpenMP {threads} threads)"}
+
+
+def _sync_time(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def ensemble_bench(steps, seed=0):
+    """64 independent 2D vortex problems (K log-uniform in [128, 8192], N in
+    {2,3,4}, strength U[3,6], centre U[-2,2]^2): the ensemble kernel with its
+    graph-replayed step against one wadges_stage launch per problem per stage."""
+    import numpy as np
+    from scratch_wadge_amd.ensemble import Ensemble, PerProblem, vortex_ensemble
+    t0 = time.perf_counter()
+    probs = vortex_ensemble(64, seed=seed)
+    ens, base, base_g = Ensemble(probs), PerProblem(probs), PerProblem(probs)
+    print(f"# ensemble setup {time.perf_counter() - t0:.1f}s K_total={ens.K_total}",
+          file=sys.stderr, flush=True)
+    tuned_ms = ens.tune(reps=5)
+    t_batch = _sync_time(lambda: ens.run(steps, use_graph=True))
+    t_base = _sync_time(lambda: base.run(steps))
+    equal = all(np.array_equal(ens.state(q)[f], base.state(q)[f]) for q in range(len(probs))
+                for f in ("Q", "rhsQ", "resQ", "Qf"))
+    finite = all(np.isfinite(ens.state(q)["Q"]).all() for q in range(len(probs)))
+    t_batch_eager = _sync_time(lambda: ens.run(steps, use_graph=False))
+    base_g.run(2, use_graph=True)                 # capture outside the timing
+    t_base_g = _sync_time(lambda: base_g.run(steps, use_graph=True))
+    work = ens.dofs * 5 * steps / 1e9
+    g = lambda t: round(work / t, 4)
+    return {"workload": "64 2D isentropic vortex problems, K log-uniform in [128, 8192] "
+                        "(nearest 2n^2), N in {2,3,4}, strength U[3,6], centre U[-2,2]^2, seed 0",
+            "K_total": ens.K_total, "steps": steps, "unit": "GDOF*stage/s",
+            "batched_graph": g(t_batch), "batched_eager": g(t_batch_eager),
+            "per_problem_eager": g(t_base), "per_problem_graph": g(t_base_g),
+            "speedup_vs_per_problem_eager": round(t_base / t_batch, 3),
+            "speedup_vs_per_problem_graph": round(t_base_g / t_batch, 3),
+            "launch_shape": ens.config(), "tuned_stage_ms": round(tuned_ms, 4),
+            "bitwise_equal_to_per_problem": bool(equal), "finite": bool(finite)}
+
+
+def adjoint_bench(steps, ckpt=40):
+    """2D vortex, 32x32x2 = 2048 triangles, N=3: density misfit after `steps`
+    LSRK45 steps against a target from Gaussian-bump-perturbed initial density;
+    times the checkpointed adjoint sweep against the plain forward."""
+    import torch
+    from scratch_wadge_amd import problem as P
+    from scratch_wadge_amd.adjoint import AdjointSolver, density_misfit, misfit_setup
+    from scratch_wadge_amd.device import DeviceSolver
+    prob = P.vortex2d_problem(n=32, N=3)
+    dev = DeviceSolver(prob)
+    Q0, _, target = misfit_setup(prob, dev, steps)
+    adj = AdjointSolver(dev, checkpoint_every=ckpt)
+    dev.run(2, fused=True)                          # warm-up
+    t_plain = _sync_time(lambda: dev.run(steps, fused=True))
+    out = {}
+    t_fwd = _sync_time(lambda: out.setdefault("QT", adj.forward(Q0, steps)))
+    QT = out["QT"].clone().requires_grad_(True)
+    loss = density_misfit(prob, QT, target)
+    loss.backward()
+    t_bwd = _sync_time(lambda: out.setdefault("g", adj.backward(QT.grad)))
+    return {"workload": f"2D vortex 32x32x2 (K={prob.K}), N=3, {steps} steps, checkpoints every {ckpt}",
+            "forward_ms": round(t_plain * 1e3, 2), "forward_with_checkpoints_ms": round(t_fwd * 1e3, 2),
+            "adjoint_sweep_ms": round(t_bwd * 1e3, 2),
+            "adjoint_over_forward": round(t_bwd / t_plain, 3),
+            "forward_plus_adjoint_over_forward": round((t_fwd + t_bwd) / t_plain, 3),
+            "loss": float(loss), "grad_norm": float(torch.linalg.norm(out["g"]))}
 
 
 def main():
@@ -213,6 +288,13 @@ def main():
     }
     if args.cpu:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
+    if args.extras:
+        del dev
+        torch.cuda.empty_cache()
+        out["ensemble"] = ensemble_bench(args.ensemble_steps)
+        print(f"# ensemble {out['ensemble']}", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_bench(args.adjoint_steps)
+        print(f"# adjoint {out['adjoint']}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -168,3 +168,41 @@ def density_misfit(prob, QT, target_rho):
     J = torch.as_tensor(prob.Jq, device=QT.device).reshape(K, Nq)
     rho = QT.reshape(K, nf, Nq)[:, 0, :]
     return 0.5 * torch.sum(wq[None, :] * J * (rho - target_rho.reshape(K, Nq)) ** 2)
+
+
+def gaussian_bumps(xq, seed=0, count=4, amp=1e-2, width=1.0, box=5.0):
+    """sum of `count` Gaussian bumps of amplitude `amp` and width `width` with
+    centres uniform in [-box/2, box/2]^dim, at the points xq (dim x ...)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    dim = len(xq)
+    out = np.zeros_like(xq[0])
+    for _ in range(count):
+        c = rng.uniform(-box / 2, box / 2, size=dim)
+        r2 = sum((xq[d] - c[d]) ** 2 for d in range(dim))
+        out = out + amp * np.exp(-r2 / (width * width))
+    return out
+
+
+def misfit_setup(prob, dev, nsteps, seed=0, amp=1e-2):
+    """Target for the density misfit: the density after nsteps from the initial
+    state whose density is multiplied by (1 + Gaussian bumps of amplitude amp),
+    at fixed velocity and pressure.  Returns (Q0, Q0_perturbed, target_rho) as
+    device tensors in the Q layout (rho layout for the target)."""
+    import numpy as np
+    from .problem import GAMMA, conservative_vars, entropy_vars  # noqa: F401
+    K, nf, Nq = prob.K, prob.nfields, prob.Nq
+    U = prob.Q.reshape(K, nf, Nq).copy()
+    bump = gaussian_bumps(prob.xq, seed=seed, amp=amp).reshape(Nq, K).T      # (K, Nq)
+    rho = U[:, 0, :]
+    vel = U[:, 1:-1, :] / rho[:, None, :]
+    p = (prob.gamma - 1.0) * (U[:, -1, :] - 0.5 * rho * (vel ** 2).sum(1))
+    rho2 = rho * (1.0 + bump)
+    U2 = np.concatenate([rho2[:, None], rho2[:, None] * vel,
+                         (p / (prob.gamma - 1.0) + 0.5 * rho2 * (vel ** 2).sum(1))[:, None]], 1)
+    Q0 = torch.as_tensor(prob.Q, device=dev.device)
+    Q0p = torch.as_tensor(U2.reshape(-1), device=dev.device)
+    adj = AdjointSolver(dev, checkpoint_every=nsteps + 1)
+    QT = adj.forward(Q0p, nsteps)
+    target = QT.reshape(K, nf, Nq)[:, 0, :].reshape(-1).clone()
+    return Q0, Q0p, target

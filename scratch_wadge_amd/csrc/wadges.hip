@@ -143,6 +143,24 @@ template <class S> struct VolSmem {
   static constexpr int per_elem = acc_v + (phase1 > phase2 ? phase1 : phase2);
 };
 
+// Waves per SIMD that the LDS of a one-wave workgroup of `bytes` allows (160 KiB
+// of LDS per CU, four SIMDs), rounded up.  Kernels whose occupancy LDS limits
+// declare it, so that the compiler may spend the registers that occupancy
+// leaves free on latency hiding (batched loads, unrolled row products) instead
+// of aiming at an occupancy the LDS cannot reach.
+constexpr int lds_waves_per_simd(int bytes) {
+  const int wg = (160 * 1024) / (bytes > 0 ? bytes : 1);
+  const int w = (wg + 3) / 4;
+  return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+template <class S> constexpr int vol_waves() {
+  return lds_waves_per_simd((WAVE / S::NQ) * VolSmem<S>::per_elem * 8 + S::NFACES * S::DIM * 8);
+}
+// Lower bound handed to the compiler: at most four waves (128 VGPRs), so that
+// shapes whose LDS would allow more are not squeezed into spills.
+template <class S> constexpr int vol_waves_min() { return vol_waves<S>() < 4 ? vol_waves<S>() : 4; }
+
 template <int DIM>
 __device__ __forceinline__ Node<DIM> load_node(const double *rec, int stride, int j) {
   Node<DIM> n;
@@ -313,7 +331,7 @@ __device__ __forceinline__ void load_nref(int lane, const double *nrJ, const dou
 }
 
 template <class S, int TPW>
-__global__ void __launch_bounds__(WAVE)
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(vol_waves_min<S>(), vol_waves<S>())))
 vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vfgeo,
            const double *__restrict__ nrJ, const double *__restrict__ nsJ,
            const double *__restrict__ ntJ, const double *__restrict__ ws,
@@ -623,7 +641,7 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   if (valid) {
     double val[NFLD];
     for (int f = 0; f < NFLD; ++f) val[f] = 0.0;
-#pragma unroll 4
+#pragma unroll 8
     for (int j = 0; j < NFN; ++j) {
       const double L = A->VqLq[i + j * NQ];
       for (int f = 0; f < NFLD; ++f) val[f] += L * accF[f * NFN + j];
@@ -644,7 +662,7 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   __syncthreads();
   auto matvec = [&](const double *in, double scale, double (&r)[NFLD]) {
     for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
-#pragma unroll 2
+#pragma unroll 8
     for (int j = 0; j < NQ; ++j) {
       const double P = scale * A->VqPq[i + j * NQ];
       for (int f = 0; f < NFLD; ++f) r[f] += P * in[f * NQ + j];
@@ -687,7 +705,7 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
     for (int fn = i; fn < NFN; fn += NQ) {
       double V[NFLD], U[NFLD];
       for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
-#pragma unroll 2
+#pragma unroll 8
       for (int j = 0; j < NQ; ++j) {
         const double P = A->VfPq[fn + j * NFN];
         for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[f * NQ + j];
@@ -772,7 +790,8 @@ diag_reduce_kernel(int K, const double *__restrict__ aux, double *__restrict__ t
 
 
 template <class S, int TPW>
-__global__ void __launch_bounds__(WAVE) stage_kernel(StageArgs A) {
+__global__ void __launch_bounds__(WAVE)
+__attribute__((amdgpu_waves_per_eu(vol_waves_min<S>(), vol_waves<S>()))) stage_kernel(StageArgs A) {
   extern __shared__ double smem[];
   __shared__ double s_nref[S::NFACES][S::DIM];
   stage_body<S, TPW>((const ConstArgs *)&A, A.fa, A.fb, A.Qf_in, A.Qf_out, blockIdx.x * TPW,

@@ -71,7 +71,7 @@ class Ensemble:
         self._qf = [(sv.s["Qf"], sv.Qf_alt) for sv in self.solvers]   # fixed buffer roles
         self.batch = _lib.Batch(self.dim, groups, problems[0].gamma, problems[0].tau, self.stream())
         self.K_total = sum(p.K for p in problems)
-        self.dofs = sum(p.K * p.Np for p in problems)
+        self.dofs = sum(p.K * p.Np * p.nfields for p in problems)   # DOFs incl. fields
 
     def stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream

@@ -1,16 +1,21 @@
-"""Discrete adjoint on the GPU: dot-product tests of each VJP kernel against
-central finite differences of the matching forward kernel, and the gradient of
-a density misfit through LSRK45 steps against finite differences."""
+"""Discrete adjoint on the GPU: each VJP kernel against the reference gradient
+(torch autograd through the torch restatement in torch_ref.py, pinned to the
+oracle) and against central finite differences of the matching forward kernel;
+the gradient of a density misfit through LSRK45 steps against both."""
 
 import numpy as np
 import pytest
 import torch
 
-from helpers import small_problem
+import torch_ref as TR
+from helpers import relerr, small_problem
 
 pytestmark = pytest.mark.gpu
 
 EPS = 1e-6
+# The update outputs are O(100) (total energy) against O(1e-2) directional
+# derivatives, so a smaller step there is dominated by round-off.
+EPS_UPDATE = 1e-5
 
 
 def _solver(prob):
@@ -96,10 +101,11 @@ def test_update_vjp_dot(dim, N):
                         _ptr(Qod), _ptr(Qb), _ptr(rb), _ptr(uqbd), _ptr(ufbd), _ptr(Rb), dev.stream())
     torch.cuda.synchronize()
     adj = float(np.dot(Qb.cpu().numpy(), dQ) + np.dot(rb.cpu().numpy(), dr) + np.dot(Rb.cpu().numpy(), dR))
-    p = fwd(Q + EPS * dQ, res + EPS * dr, R + EPS * dR)
-    m = fwd(Q - EPS * dQ, res - EPS * dr, R - EPS * dR)
+    e = EPS_UPDATE
+    p = fwd(Q + e * dQ, res + e * dr, R + e * dR)
+    m = fwd(Q - e * dQ, res - e * dr, R - e * dR)
     cots = (Qb_o, rb_o, uqb_o, ufb_o)
-    fd = float(sum(np.dot(cb, (a - b) / (2 * EPS)) for cb, a, b in zip(cots, p, m)))
+    fd = float(sum(np.dot(cb, (a - b) / (2 * e)) for cb, a, b in zip(cots, p, m)))
     print("update vjp: adjoint", adj, "fd", fd, "rel", _rel(adj, fd), flush=True)
     assert _rel(adj, fd) < 1e-6
 
@@ -138,3 +144,90 @@ def test_trajectory_gradient_fd(dim, N, steps, ckpt):
         worst = max(worst, _rel(float(g @ d), fd))
         print("trajectory: adjoint", float(g @ d), "fd", fd, flush=True)
     assert worst < 1e-6
+
+
+# ----------------------------------------------------------------- reference gradient
+
+@pytest.mark.parametrize("dim,N", [(2, 2), (2, 3), (3, 3), (3, 4)])
+def test_rhs_vjp_matches_reference_gradient(dim, N):
+    from scratch_wadge_amd.device import _ptr
+    prob = small_problem(dim, N)
+    print(f"case rhs vjp vs autograd dim={dim} N={N} K={prob.K}", flush=True)
+    dev = _solver(prob)
+    rng = np.random.default_rng(5)
+    uq = prob.rhsQ * (1.0 + 1e-2 * rng.standard_normal(prob.rhsQ.shape))
+    uf = prob.Qf * (1.0 + 1e-2 * rng.standard_normal(prob.Qf.shape))
+    Rb = rng.standard_normal(uq.shape)
+    c = dev.c
+    uqd, ufd, Rbd = _dev(uq), _dev(uf), _dev(Rb)
+    sbar = torch.empty_like(ufd)
+    uqb, ufb = torch.empty_like(uqd), torch.empty_like(ufd)
+    dev.plan.rhs_vjp(prob.K, _ptr(c["vgeo"]), _ptr(c["vfgeo"]), _ptr(c["fgeo"]), _ptr(c["mapPq"]),
+                     _ptr(c["nrJ"]), _ptr(c["nsJ"]), _ptr(c["ntJ"]), _ptr(c["Drq"]), _ptr(c["Dsq"]),
+                     _ptr(c["Dtq"]), _ptr(c["VqLq"]), _ptr(c["VfPq"]), _ptr(uqd), _ptr(ufd),
+                     _ptr(Rbd), _ptr(sbar), _ptr(uqb), _ptr(ufb), dev.stream())
+    ops = TR.Ops(prob, "cuda:0")
+    a = uqd.clone().requires_grad_(True)
+    b = ufd.clone().requires_grad_(True)
+    (TR.rhs(ops, a, b) * Rbd).sum().backward()
+    torch.cuda.synchronize()
+    e1 = relerr(uqb.cpu().numpy(), a.grad.cpu().numpy())
+    e2 = relerr(ufb.cpu().numpy(), b.grad.cpu().numpy())
+    print(f"rhs vjp vs autograd: volume {e1:.2e} face {e2:.2e}", flush=True)
+    assert e1 < 1e-11 and e2 < 1e-11
+
+
+@pytest.mark.parametrize("dim,N", [(2, 3), (3, 3)])
+def test_update_vjp_matches_reference_gradient(dim, N):
+    from scratch_wadge_amd.device import _ptr
+    from scratch_wadge_amd.problem import RK4A, RK4B
+    prob = small_problem(dim, N)
+    print(f"case update vjp vs autograd dim={dim} N={N}", flush=True)
+    dev = _solver(prob)
+    rng = np.random.default_rng(6)
+    Q = prob.Q.copy()
+    R = 0.1 * rng.standard_normal(Q.shape)
+    res = 1e-3 * rng.standard_normal(Q.shape)
+    fa, fb, dt = RK4A[1], RK4B[1], 2e-3
+    cots = [rng.standard_normal(Q.shape) for _ in range(3)] + [rng.standard_normal(prob.Qf.shape)]
+    dev.set_state(Q=Q, resQ=res, rhsQ=R, Qf=prob.Qf)
+    dev.update(fa, fb, dt)
+    Qo = dev.s["Q"].clone()
+    c = dev.c
+    Qb, rb = _dev(cots[0]), _dev(cots[1])
+    uqb, ufb = _dev(cots[2]), _dev(cots[3])
+    Rb = torch.empty_like(Qb)
+    dev.plan.update_vjp(prob.K, _ptr(c["Jq"]), _ptr(c["VqPq"]), _ptr(c["VfPq"]), fa, fb, dt,
+                        _ptr(Qo), _ptr(Qb), _ptr(rb), _ptr(uqb), _ptr(ufb), _ptr(Rb), dev.stream())
+    ops = TR.Ops(prob, "cuda:0")
+    Qt, rt, Rt = (_dev(x).requires_grad_(True) for x in (Q, res, R))
+    outs = TR.update(ops, fa, fb, dt, Rt, rt, Qt)
+    sum((o * _dev(cb)).sum() for o, cb in zip(outs, cots)).backward()
+    torch.cuda.synchronize()
+    errs = [relerr(k.cpu().numpy(), t.grad.cpu().numpy()) for k, t in ((Qb, Qt), (rb, rt), (Rb, Rt))]
+    print("update vjp vs autograd:", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) < 1e-12
+
+
+@pytest.mark.parametrize("dim,N,steps,ckpt", [(2, 3, 4, 3), (3, 3, 2, 1)])
+def test_trajectory_matches_reference_gradient(dim, N, steps, ckpt):
+    """The checkpointed discrete adjoint equals autograd through the torch
+    restatement of the same LSRK45 steps."""
+    from scratch_wadge_amd.adjoint import AdjointSolver, RKRollout, density_misfit
+    from scratch_wadge_amd.problem import RK4A, RK4B
+    prob = small_problem(dim, N)
+    print(f"case trajectory vs autograd dim={dim} N={N} steps={steps}", flush=True)
+    dev = _solver(prob)
+    adj = AdjointSolver(dev, checkpoint_every=ckpt)
+    rng = np.random.default_rng(7)
+    Q0 = _dev(prob.Q * (1.0 + 1e-2 * rng.standard_normal(prob.Q.shape)))
+    target = _dev(prob.Q.reshape(prob.K, prob.nfields, prob.Nq)[:, 0, :].reshape(-1))
+    q = Q0.clone().requires_grad_(True)
+    density_misfit(prob, RKRollout.apply(q, adj, steps), target).backward()
+    ops = TR.Ops(prob, "cuda:0")
+    qt = Q0.clone().requires_grad_(True)
+    density_misfit(prob, TR.run(ops, qt, steps, prob.dt, RK4A, RK4B), target).backward()
+    torch.cuda.synchronize()
+    e = relerr(q.grad.cpu().numpy(), qt.grad.cpu().numpy())
+    print(f"trajectory gradient vs autograd: {e:.2e}", flush=True)
+    assert e < 1e-10
