@@ -42,6 +42,8 @@ def parse():
                     help="also measure the ensemble and adjoint workloads (default)")
     ap.add_argument("--no-extras", dest="extras", action="store_false")
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--leg", choices=("all", "ensemble", "adjoint"), default="all",
+                    help="ensemble / adjoint: run only that leg and print its JSON object")
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="instead of the benchmark, copy a GIB-sized device buffer 5 times "
@@ -149,7 +151,8 @@ def ensemble_bench(steps, seed=0):
 def adjoint_bench(steps, ckpt=40):
     """2D vortex, 32x32x2 = 2048 triangles, N=3: density misfit after `steps`
     LSRK45 steps against a target from Gaussian-bump-perturbed initial density;
-    times the checkpointed adjoint sweep against the plain forward."""
+    times the checkpointed adjoint sweep against the plain forward (both
+    replayed from HIP graphs)."""
     import torch
     from scratch_wadge_amd import problem as P
     from scratch_wadge_amd.adjoint import AdjointSolver, density_misfit, misfit_setup
@@ -158,20 +161,23 @@ def adjoint_bench(steps, ckpt=40):
     dev = DeviceSolver(prob)
     Q0, _, target = misfit_setup(prob, dev, steps)
     adj = AdjointSolver(dev, checkpoint_every=ckpt)
-    dev.run(2, fused=True)                          # warm-up
-    t_plain = _sync_time(lambda: dev.run(steps, fused=True))
+    dev.run(4, fused=True, graph=True)              # warm-up, graph capture
+    t_plain = _sync_time(lambda: dev.run(steps, fused=True, graph=True))
+    adj.forward(Q0, 2 * ckpt)                       # warm-up, graph captures
+    adj.backward(torch.zeros_like(Q0))
     out = {}
     t_fwd = _sync_time(lambda: out.setdefault("QT", adj.forward(Q0, steps)))
     QT = out["QT"].clone().requires_grad_(True)
     loss = density_misfit(prob, QT, target)
     loss.backward()
+    loss = float(loss.detach())
     t_bwd = _sync_time(lambda: out.setdefault("g", adj.backward(QT.grad)))
     return {"workload": f"2D vortex 32x32x2 (K={prob.K}), N=3, {steps} steps, checkpoints every {ckpt}",
             "forward_ms": round(t_plain * 1e3, 2), "forward_with_checkpoints_ms": round(t_fwd * 1e3, 2),
             "adjoint_sweep_ms": round(t_bwd * 1e3, 2),
             "adjoint_over_forward": round(t_bwd / t_plain, 3),
             "forward_plus_adjoint_over_forward": round((t_fwd + t_bwd) / t_plain, 3),
-            "loss": float(loss), "grad_norm": float(torch.linalg.norm(out["g"]))}
+            "loss": loss, "grad_norm": float(torch.linalg.norm(out["g"]))}
 
 
 def main():
@@ -183,6 +189,12 @@ def main():
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the element "
               f"partition across GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
+    if args.leg == "ensemble":
+        print(json.dumps({"ensemble": ensemble_bench(args.ensemble_steps)}), flush=True)
+        return
+    if args.leg == "adjoint":
+        print(json.dumps({"adjoint": adjoint_bench(args.adjoint_steps)}), flush=True)
+        return
     import numpy as np
     import torch
 

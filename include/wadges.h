@@ -116,8 +116,43 @@ int wadges_stage(const wadges_plan *plan, int K, const double *vgeo, const doubl
                  double fa, double fb, double fdt, const double *Qf_in, double *Qf_out,
                  double *rhsQ, double *resQ, double *Q, void *stream);
 
+/* wadges_stage with separate input and output arrays for the projected state,
+ * the RK residual and the state (each output may equal its input; Qf_out must
+ * differ from Qf_in).  Same arithmetic as wadges_stage; it lets a caller keep
+ * every stage of a trajectory without copies (the adjoint's recomputation).
+ * No counterpart in the reference. */
+int wadges_stage_record(const wadges_plan *plan, int K, const double *vgeo, const double *vfgeo,
+                        const double *fgeo, const uint32_t *mapPq, const double *nrJ,
+                        const double *nsJ, const double *ntJ, const double *Drq, const double *Dsq,
+                        const double *Dtq, const double *VqLq, const double *VqPq,
+                        const double *VfPq, const double *Jq, double fa, double fb, double fdt,
+                        const double *Qf_in, double *Qf_out, const double *rhsQ_in,
+                        double *rhsQ_out, const double *resQ_in, double *resQ_out,
+                        const double *Q_in, double *Q_out, void *stream);
+
 /* ---- Discrete adjoint (vector-Jacobian products); no counterpart in the
  * reference.  Cotangent arrays use the layout of their primal arrays.
+ *
+ * wadges_stage_vjp: VJP of one whole stage (wadges_stage or wadges_stage_record
+ * with coefficients fa, fb, fdt) in two launches: wadges_update_vjp with the
+ * transposed lift s_bar = VqLq^T R_bar fused in, then the RHS VJP.  rhsQ_in,
+ * Qf_in are the stage's input projected state, Q_out its output state.  On
+ * entry Q_bar, resQ_bar, rhsQ_bar_out, Qf_bar_out are the cotangents of the
+ * stage outputs; on return Q_bar, resQ_bar are those of the stage inputs Q,
+ * resQ and rhsQ_bar_in, Qf_bar_in those of rhsQ_in, Qf_in.  R_bar (K*Nfields*Nq)
+ * and sbar (K*Nfields*Nf) are caller scratch.  rhsQ_bar_in may equal
+ * rhsQ_bar_out and Qf_bar_in may equal Qf_bar_out (the second launch writes
+ * them after the first has read them). */
+int wadges_stage_vjp(const wadges_plan *plan, int K, const double *vgeo, const double *vfgeo,
+                     const double *fgeo, const uint32_t *mapPq, const double *nrJ,
+                     const double *nsJ, const double *ntJ, const double *Drq, const double *Dsq,
+                     const double *Dtq, const double *VqLq, const double *VqPq, const double *VfPq,
+                     const double *Jq, double fa, double fb, double fdt, const double *rhsQ_in,
+                     const double *Qf_in, const double *Q_out, double *Q_bar, double *resQ_bar,
+                     const double *rhsQ_bar_out, const double *Qf_bar_out, double *R_bar,
+                     double *sbar, double *rhsQ_bar_in, double *Qf_bar_in, void *stream);
+
+/* The single-kernel VJPs the stage VJP is made of:
  *
  * wadges_rhs_vjp: given rhsQ_bar, the cotangent of the stage RHS (the rhsQ that
  * vol -> surf produce from the projected state rhsQ and traces Qf), writes the

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libwadges.so")
 # Every symbol the header declares (checked by the CPU tests).
 EXPORTED = (
     "wadges_plan_create", "wadges_plan_destroy", "wadges_plan_invalidate", "wadges_vol", "wadges_surf", "wadges_update",
-    "wadges_stage", "wadges_diagnostics", "wadges_rhs_vjp", "wadges_update_vjp",
+    "wadges_stage", "wadges_stage_record", "wadges_stage_vjp", "wadges_diagnostics", "wadges_rhs_vjp", "wadges_update_vjp",
     "wadges_project_vjp", "wadges_batch_create", "wadges_batch_destroy", "wadges_batch_stage",
     "wadges_batch_run", "wadges_batch_current", "wadges_batch_get_config", "wadges_batch_set_config",
     "wadges_batch_tune",
@@ -65,6 +65,10 @@ def load():
     L.wadges_update.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, _vp, _vp, _vp, _vp, _vp]
     L.wadges_stage.argtypes = [_vp, ctypes.c_int] + [_vp] * 14 + [ctypes.c_double] * 3 + [_vp] * 5 + [_vp]
+    L.wadges_stage_record.argtypes = ([_vp, ctypes.c_int] + [_vp] * 14 + [ctypes.c_double] * 3 +
+                                      [_vp] * 8 + [_vp])
+    L.wadges_stage_vjp.argtypes = ([_vp, ctypes.c_int] + [_vp] * 14 + [ctypes.c_double] * 3 +
+                                   [_vp] * 11 + [_vp])
     L.wadges_diagnostics.argtypes = [_vp, ctypes.c_int] + [_vp] * 6 + [_vp]
     L.wadges_rhs_vjp.argtypes = [_vp, ctypes.c_int] + [_vp] * 18 + [_vp]
     L.wadges_update_vjp.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
@@ -140,6 +144,22 @@ class Plan:
         check(load().wadges_stage(self.h, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq,
                                   VqLq, VqPq, VfPq, Jq, fa, fb, fdt, Qf_in, Qf_out, rhsQ, resQ, Q,
                                   stream), "wadges_stage")
+
+    def stage_record(self, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, VqLq, VqPq,
+                     VfPq, Jq, fa, fb, fdt, Qf_in, Qf_out, rhsQ_in, rhsQ_out, resQ_in, resQ_out,
+                     Q_in, Q_out, stream=None):
+        check(load().wadges_stage_record(self.h, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq,
+                                         Dsq, Dtq, VqLq, VqPq, VfPq, Jq, fa, fb, fdt, Qf_in, Qf_out,
+                                         rhsQ_in, rhsQ_out, resQ_in, resQ_out, Q_in, Q_out, stream),
+              "wadges_stage_record")
+
+    def stage_vjp(self, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, VqLq, VqPq, VfPq,
+                  Jq, fa, fb, fdt, rhsQ_in, Qf_in, Q_out, Q_bar, resQ_bar, rhsQ_bar_out, Qf_bar_out,
+                  R_bar, sbar, rhsQ_bar_in, Qf_bar_in, stream=None):
+        check(load().wadges_stage_vjp(self.h, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq,
+                                      Dtq, VqLq, VqPq, VfPq, Jq, fa, fb, fdt, rhsQ_in, Qf_in, Q_out,
+                                      Q_bar, resQ_bar, rhsQ_bar_out, Qf_bar_out, R_bar, sbar,
+                                      rhsQ_bar_in, Qf_bar_in, stream), "wadges_stage_vjp")
 
     def diagnostics(self, K, wq, Jq, Q, Qexact, aux, totals, stream=None):
         check(load().wadges_diagnostics(self.h, K, wq, Jq, Q, Qexact, aux, totals, stream),

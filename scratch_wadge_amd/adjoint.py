@@ -7,11 +7,12 @@ conservative state Q0 at the volume quadrature points:
     resQ = 0
     repeat nsteps x 5 stages: vol -> surf -> update
 
-Its reverse sweep chains the hand-written VJP kernels (wadges_update_vjp,
-wadges_rhs_vjp, wadges_project_vjp).  Memory follows the usual checkpointing
+Its reverse sweep chains the hand-written VJP kernels (wadges_stage_vjp per
+stage, wadges_project_vjp at the end).  Memory follows the usual checkpointing
 scheme: the forward pass keeps the full state every `checkpoint_every` steps;
-the reverse pass re-runs each segment from its checkpoint, keeping the stage
-states of that segment only, then walks the segment's stages backwards.
+the reverse pass re-runs each segment from its checkpoint with
+wadges_stage_record, which writes every stage's state into its own slot (no
+copies), then walks the segment's stages backwards.
 
 `RKRollout` wraps the forward map as a torch.autograd.Function whose backward is
 this adjoint.
@@ -26,20 +27,35 @@ from .problem import RK4A, RK4B
 
 
 class AdjointSolver:
-    def __init__(self, dev: DeviceSolver, checkpoint_every=40, fused=True):
+    """Checkpointed discrete adjoint of LSRK45 on one DeviceSolver.
+
+    Memory: the forward keeps (Q, resQ, rhsQ, Qf) every `checkpoint_every`
+    steps; the reverse pass keeps one segment's stage states, (5C + 1) x
+    (|Q| + |rhsQ| + |Qf|) doubles for C = checkpoint_every.  With use_graph the
+    forward steps and each segment's recomputation + reverse sweep are replayed
+    from HIP graphs (captured through torch.cuda.graph on first use)."""
+
+    def __init__(self, dev: DeviceSolver, checkpoint_every=40, fused=True, use_graph=True):
         self.dev = dev
         self.C = max(1, int(checkpoint_every))
         self.fused = fused
+        self.use_graph = use_graph
         s = dev.s
         self.sbar = torch.empty_like(s["Qf"])
         self.Rb = torch.empty_like(s["Q"])
+        self.Qb = torch.empty_like(s["Q"])
+        self.resb = torch.empty_like(s["Q"])
+        self.uqb = torch.empty_like(s["rhsQ"])
+        self.ufb = torch.empty_like(s["Qf"])
+        self.res_work = torch.empty_like(s["Q"])
+        self._slots = None
+        self._seg_graphs = {}
         self.stats = {}
 
     # ------------------------------------------------------------------ forward
     def _project(self, Q0):
         """Initial projection rhsQ = u(VqPq v(Q0)), Qf = u(VfPq v(Q0)) on the host
         side of the driver, like the reference (euler.c:4028-4133)."""
-        from .problem import conservative_vars, entropy_vars
         prob = self.dev.prob
         K, nf, Nq = prob.K, prob.nfields, prob.Nq
         U = Q0.detach().reshape(K, nf, Nq).permute(1, 2, 0)          # (nf, Nq, K)
@@ -60,19 +76,9 @@ class AdjointSolver:
         d.s["rhsQ"].copy_(uq)
         d.s["Qf"].copy_(uf)
 
-    def _stage(self, s, dt):
-        if self.fused:
-            self.dev.stage_fused(s, dt)
-        else:
-            self.dev.stage(s, dt)
-
     def _snapshot(self):
         s = self.dev.s
         return {k: s[k].clone() for k in ("Q", "resQ", "rhsQ", "Qf")}
-
-    def _restore(self, snap):
-        for k, v in snap.items():
-            self.dev.s[k].copy_(v)
 
     def forward(self, Q0, nsteps, dt=None):
         """Runs nsteps LSRK45 steps from Q0 (device tensor in the Q layout);
@@ -82,49 +88,72 @@ class AdjointSolver:
         self.Q0 = Q0.detach().clone()
         self._set_initial(self.Q0)
         self.ckpt = {}
-        for step in range(nsteps):
-            if step % self.C == 0:
-                self.ckpt[step] = self._snapshot()
-            for s in range(5):
-                self._stage(s, dt)
+        for seg in range(0, nsteps, self.C):
+            self.ckpt[seg] = self._snapshot()
+            self.dev.run(min(self.C, nsteps - seg), dt, fused=self.fused, graph=self.use_graph)
         return self.dev.s["Q"].clone()
 
     # ------------------------------------------------------------------ reverse
+    def _ensure_slots(self):
+        if self._slots is None:
+            L = 5 * self.C + 1
+            s = self.dev.s
+            mk = lambda t: torch.empty((L, t.numel()), dtype=t.dtype, device=t.device)
+            self._slots = {"Q": mk(s["Q"]), "rhsQ": mk(s["rhsQ"]), "Qf": mk(s["Qf"])}
+        return self._slots
+
+    def _segment(self, nst, dt):
+        """Recompute nst stages from slot 0 into the slots, then walk them back."""
+        d, c = self.dev, self.dev.c
+        plan, K, st = d.plan, d.K, d.stream()
+        S = self._slots
+        P = lambda n: _ptr(c[n])
+        geo = (P("vgeo"), P("vfgeo"), P("fgeo"), P("mapPq"), P("nrJ"), P("nsJ"), P("ntJ"), P("Drq"),
+               P("Dsq"), P("Dtq"), P("VqLq"), P("VqPq"), P("VfPq"), P("Jq"))
+        res = _ptr(self.res_work)
+        for idx in range(nst):
+            s = idx % 5
+            plan.stage_record(K, *geo, RK4A[s], RK4B[s], dt, _ptr(S["Qf"][idx]), _ptr(S["Qf"][idx + 1]),
+                              _ptr(S["rhsQ"][idx]), _ptr(S["rhsQ"][idx + 1]), res, res,
+                              _ptr(S["Q"][idx]), _ptr(S["Q"][idx + 1]), st)
+        uqb, ufb = _ptr(self.uqb), _ptr(self.ufb)
+        for idx in range(nst - 1, -1, -1):
+            s = idx % 5
+            plan.stage_vjp(K, *geo, RK4A[s], RK4B[s], dt, _ptr(S["rhsQ"][idx]), _ptr(S["Qf"][idx]),
+                           _ptr(S["Q"][idx + 1]), _ptr(self.Qb), _ptr(self.resb), uqb, ufb,
+                           _ptr(self.Rb), _ptr(self.sbar), uqb, ufb, st)
+
+    def _run_segment(self, nst, dt):
+        if not self.use_graph:
+            self._segment(nst, dt)
+            return
+        key = (nst, dt)
+        g = self._seg_graphs.get(key)
+        if g is None:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._segment(nst, dt)
+            self._seg_graphs[key] = g
+        g.replay()
+
     def backward(self, Q_bar_T):
         """Gradient of <Q_bar_T, Q(T)> with respect to Q0."""
-        d, c, prob = self.dev, self.dev.c, self.dev.prob
-        plan, K, st = d.plan, d.K, d.stream()
-        Qb = Q_bar_T.detach().reshape(-1).clone()
-        resb = torch.zeros_like(Qb)
-        uqb = torch.zeros_like(d.s["rhsQ"])
-        ufb = torch.zeros_like(d.s["Qf"])
-        uqb_new = torch.zeros_like(uqb)
-        ufb_new = torch.zeros_like(ufb)
-        dt = self.dt
+        d, c = self.dev, self.dev.c
+        S = self._ensure_slots()
+        self.Qb.copy_(Q_bar_T.detach().reshape(-1))
+        self.resb.zero_()
+        self.uqb.zero_()
+        self.ufb.zero_()
         for seg in sorted(self.ckpt, reverse=True):
-            end = min(seg + self.C, self.nsteps)
-            self._restore(self.ckpt[seg])
-            states = [self._snapshot()]       # state before each stage, then the end state
-            for step in range(seg, end):
-                for s in range(5):
-                    self._stage(s, dt)
-                    states.append(self._snapshot())
-            for idx in range(len(states) - 2, -1, -1):
-                s = idx % 5
-                out, inp = states[idx + 1], states[idx]
-                plan.update_vjp(K, _ptr(c["Jq"]), _ptr(c["VqPq"]), _ptr(c["VfPq"]), RK4A[s], RK4B[s],
-                                dt, _ptr(out["Q"]), _ptr(Qb), _ptr(resb), _ptr(uqb), _ptr(ufb),
-                                _ptr(self.Rb), st)
-                plan.rhs_vjp(K, _ptr(c["vgeo"]), _ptr(c["vfgeo"]), _ptr(c["fgeo"]), _ptr(c["mapPq"]),
-                             _ptr(c["nrJ"]), _ptr(c["nsJ"]), _ptr(c["ntJ"]), _ptr(c["Drq"]),
-                             _ptr(c["Dsq"]), _ptr(c["Dtq"]), _ptr(c["VqLq"]), _ptr(c["VfPq"]),
-                             _ptr(inp["rhsQ"]), _ptr(inp["Qf"]), _ptr(self.Rb), _ptr(self.sbar),
-                             _ptr(uqb_new), _ptr(ufb_new), st)
-                uqb, uqb_new = uqb_new, uqb
-                ufb, ufb_new = ufb_new, ufb
-            del states
-        plan.project_vjp(K, _ptr(c["VqPq"]), _ptr(c["VfPq"]), _ptr(self.Q0), _ptr(uqb), _ptr(ufb),
-                         _ptr(Qb), st)
+            ck = self.ckpt[seg]
+            S["Q"][0].copy_(ck["Q"])
+            S["rhsQ"][0].copy_(ck["rhsQ"])
+            S["Qf"][0].copy_(ck["Qf"])
+            self.res_work.copy_(ck["resQ"])
+            self._run_segment(5 * (min(seg + self.C, self.nsteps) - seg), self.dt)
+        Qb = self.Qb.clone()
+        d.plan.project_vjp(d.K, _ptr(c["VqPq"]), _ptr(c["VfPq"]), _ptr(self.Q0), _ptr(self.uqb),
+                           _ptr(self.ufb), _ptr(Qb), d.stream())
         return Qb
 
 

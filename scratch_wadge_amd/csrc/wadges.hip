@@ -566,7 +566,8 @@ struct StageArgs {
   const double *nrJ, *nsJ, *ntJ, *ws, *VqLq, *VqPq, *VfPq, *Jq;
   double fa, fb, fdt;
   const double *Qf_in;
-  double *Qf_out, *rhsQ, *resQ, *Q;
+  double *Qf_out, *rhsQ, *resQ, *Q;            // outputs
+  const double *rhsQ_in, *resQ_in, *Q_in;      // inputs (equal to the outputs: in place)
   double gamma, gm1, inv2gm1, ltau, lgm1;
 };
 
@@ -604,7 +605,7 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   double *accV = smem + (active ? team : 0) * PER;
   double *accF = accV + NFLD * NQ + FREC * NFN;
   double acc[NFLD];
-  vol_body<S>(valid, i, A->rhsQ + base, Qf_in + kk * NFN * NFLD, A->vgeo + kk * NQ * NG,
+  vol_body<S>(valid, i, A->rhsQ_in + base, Qf_in + kk * NFN * NFLD, A->vgeo + kk * NQ * NG,
               A->vfgeo + kk * NFN * NG, s_nref, A->ws, accV, gm1, inv2gm1, acc);
 
   // ---------------- surface flux + LF penalty, in place over the face rows
@@ -679,9 +680,9 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
     matvec(bufB, 1.0, r);
     for (int f = 0; f < NFLD; ++f) {
       const size_t id = base + i + f * NQ;
-      const double res = fa * A->resQ[id] + fdt * r[f];
+      const double res = fa * A->resQ_in[id] + fdt * r[f];
       A->resQ[id] = res;
-      U[f] = A->Q[id] + fb * res;
+      U[f] = A->Q_in[id] + fb * res;
       A->Q[id] = U[f];
     }
     entropy_vars<DIM>(U, gamma, V);
@@ -823,8 +824,16 @@ struct Ops {
   int (*stage)(const wadges_plan *, int, const double *, const double *, const double *,
                const uint32_t *, const double *, const double *, const double *, const double *,
                const double *, const double *, const double *, const double *, const double *,
-               const double *, double, double, double, const double *, double *, double *,
-               double *, double *, hipStream_t);
+               const double *, double, double, double, const double *, double *,
+               const double *, double *, const double *, double *, const double *, double *,
+               hipStream_t);
+  int (*stage_vjp)(const wadges_plan *, int, const double *, const double *, const double *,
+                   const uint32_t *, const double *, const double *, const double *,
+                   const double *, const double *, const double *, const double *,
+                   const double *, const double *, const double *, double, double, double,
+                   const double *, const double *, const double *, double *, double *,
+                   const double *, const double *, double *, double *, double *, double *,
+                   hipStream_t);
   int (*diag)(const wadges_plan *, int, const double *, const double *, const double *,
               const double *, double *, double *, hipStream_t);
   int (*rhs_vjp)(const wadges_plan *, int, const double *, const double *, const double *,
@@ -933,8 +942,9 @@ template <class S> int stage_impl(const wadges_plan *p, int K, const double *vge
                                   const double *Drq, const double *Dsq, const double *Dtq,
                                   const double *VqLq, const double *VqPq, const double *VfPq,
                                   const double *Jq, double fa, double fb, double fdt,
-                                  const double *Qf_in, double *Qf_out, double *rhsQ, double *resQ,
-                                  double *Q, hipStream_t st) {
+                                  const double *Qf_in, double *Qf_out, const double *rhsQ_in,
+                                  double *rhsQ, const double *resQ_in, double *resQ,
+                                  const double *Q_in, double *Q, hipStream_t st) {
   if (K == 0) return WADGES_OK;
   int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
   if (rc) return rc;
@@ -942,9 +952,10 @@ template <class S> int stage_impl(const wadges_plan *p, int K, const double *vge
   const size_t smem = (size_t)TPW * VolSmem<S>::per_elem * sizeof(double);
   const int blocks = (K + TPW - 1) / TPW;
   const double g = p->gamma;
-  StageArgs A{K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, p->ws, VqLq, VqPq, VfPq, Jq, fa, fb,
-              fdt, Qf_in, Qf_out, rhsQ, resQ, Q, g, g - 1.0, 1.0 / (2.0 * (g - 1.0)),
-              0.25 * p->tau, std::log(g - 1.0)};
+  StageArgs A{K,      vgeo,   vfgeo,   fgeo,   mapPq,  nrJ,     nsJ,  ntJ,
+              p->ws,  VqLq,   VqPq,    VfPq,   Jq,     fa,      fb,   fdt,
+              Qf_in,  Qf_out, rhsQ,    resQ,   Q,      rhsQ_in, resQ_in, Q_in,
+              g,      g - 1.0, 1.0 / (2.0 * (g - 1.0)), 0.25 * p->tau, std::log(g - 1.0)};
   stage_kernel<S, TPW><<<blocks, WAVE, smem, st>>>(A);
   return hip_check(hipGetLastError());
 }
@@ -998,7 +1009,41 @@ template <class S> int update_vjp_impl(const wadges_plan *p, int K, const double
   constexpr int TPW = WAVE / S::NQ;
   constexpr int EPB = 4 * TPW;
   update_vjp_kernel<S, TPW><<<(K + EPB - 1) / EPB, WAVE * 4, 0, st>>>(
-      K, Jq, VqPq, VfPq, fa, fb, fdt, Q, Qb, resb, uqb, ufb, Rb, p->gamma, std::log(p->gamma - 1.0));
+      K, Jq, VqPq, VfPq, fa, fb, fdt, Q, Qb, resb, uqb, ufb, Rb, p->gamma, std::log(p->gamma - 1.0),
+      nullptr, nullptr);
+  return hip_check(hipGetLastError());
+}
+
+// VJP of one fused stage: update VJP with the transposed lift (s_bar) fused in,
+// then the RHS VJP at the stage's input projected state.  Two launches.
+template <class S> int stage_vjp_impl(const wadges_plan *p, int K, const double *vgeo,
+                                      const double *vfgeo, const double *fgeo,
+                                      const uint32_t *mapPq, const double *nrJ, const double *nsJ,
+                                      const double *ntJ, const double *Drq, const double *Dsq,
+                                      const double *Dtq, const double *VqLq, const double *VqPq,
+                                      const double *VfPq, const double *Jq, double fa, double fb,
+                                      double fdt, const double *rhsQ_in, const double *Qf_in,
+                                      const double *Q_out, double *Qb, double *resb,
+                                      const double *uqb_out, const double *ufb_out, double *Rb,
+                                      double *sbar, double *uqb_in, double *ufb_in,
+                                      hipStream_t st) {
+  if (K == 0) return WADGES_OK;
+  int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
+  if (rc) return rc;
+  const double g = p->gamma;
+  {
+    constexpr int TPW = WAVE / S::NQ;
+    constexpr int EPB = 4 * TPW;
+    update_vjp_kernel<S, TPW><<<(K + EPB - 1) / EPB, WAVE * 4, 0, st>>>(
+        K, Jq, VqPq, VfPq, fa, fb, fdt, Q_out, Qb, resb, uqb_out, ufb_out, Rb, g,
+        std::log(g - 1.0), VqLq, sbar);
+    if ((rc = hip_check(hipGetLastError()))) return rc;
+  }
+  constexpr int TPW = WAVE / S::NQ;
+  const size_t smem = (size_t)TPW * VjpSmem<S>::per_elem * sizeof(double);
+  rhs_vjp_kernel<S, TPW><<<(K + TPW - 1) / TPW, WAVE, smem, st>>>(
+      K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, p->ws, rhsQ_in, Qf_in, Rb, sbar, uqb_in, ufb_in,
+      g, g - 1.0, 1.0 / (2.0 * (g - 1.0)), 0.25 * p->tau);
   return hip_check(hipGetLastError());
 }
 
@@ -1032,7 +1077,8 @@ struct Instance {
 template <int DIM, int NQ, int NFQ> constexpr Instance make_instance(int N) {
   using S = Shape<DIM, NQ, NFQ>;
   return Instance{DIM, N, NQ, NFQ,
-                  Ops{&vol_impl<S>, &surf_impl<S>, &update_impl<S>, &stage_impl<S>, &diag_impl<S>,
+                  Ops{&vol_impl<S>, &surf_impl<S>, &update_impl<S>, &stage_impl<S>,
+                      &stage_vjp_impl<S>, &diag_impl<S>,
                       &rhs_vjp_impl<S>, &update_vjp_impl<S>, &proj_vjp_impl<S>, &prep_impl<S>,
                       S::WS, stage_smem_bytes<S>()}};
 }
@@ -1221,7 +1267,48 @@ int wadges_stage(const wadges_plan *p, int K, const double *vgeo, const double *
   if (K > 0 && p->dim == 3 && (!ntJ || !Dtq)) return WADGES_EINVAL;
   if (K > 0 && Qf_in == Qf_out) return WADGES_EINVAL;  // neighbours read Qf_in concurrently
   return p->ops.stage(p, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, VqLq, VqPq,
-                      VfPq, Jq, fa, fb, fdt, Qf_in, Qf_out, rhsQ, resQ, Q, (hipStream_t)stream);
+                      VfPq, Jq, fa, fb, fdt, Qf_in, Qf_out, rhsQ, rhsQ, resQ, resQ, Q, Q,
+                      (hipStream_t)stream);
+}
+
+int wadges_stage_record(const wadges_plan *p, int K, const double *vgeo, const double *vfgeo,
+                        const double *fgeo, const uint32_t *mapPq, const double *nrJ,
+                        const double *nsJ, const double *ntJ, const double *Drq, const double *Dsq,
+                        const double *Dtq, const double *VqLq, const double *VqPq,
+                        const double *VfPq, const double *Jq, double fa, double fb, double fdt,
+                        const double *Qf_in, double *Qf_out, const double *rhsQ_in,
+                        double *rhsQ_out, const double *resQ_in, double *resQ_out,
+                        const double *Q_in, double *Q_out, void *stream) {
+  if (!p || K < 0) return WADGES_EINVAL;
+  if (K > 0 && (!vgeo || !vfgeo || !fgeo || !mapPq || !nrJ || !nsJ || !Drq || !Dsq || !VqLq ||
+                !VqPq || !VfPq || !Jq || !Qf_in || !Qf_out || !rhsQ_in || !rhsQ_out ||
+                !resQ_in || !resQ_out || !Q_in || !Q_out))
+    return WADGES_EINVAL;
+  if (K > 0 && p->dim == 3 && (!ntJ || !Dtq)) return WADGES_EINVAL;
+  if (K > 0 && Qf_in == Qf_out) return WADGES_EINVAL;
+  return p->ops.stage(p, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, VqLq, VqPq,
+                      VfPq, Jq, fa, fb, fdt, Qf_in, Qf_out, rhsQ_in, rhsQ_out, resQ_in, resQ_out,
+                      Q_in, Q_out, (hipStream_t)stream);
+}
+
+int wadges_stage_vjp(const wadges_plan *p, int K, const double *vgeo, const double *vfgeo,
+                     const double *fgeo, const uint32_t *mapPq, const double *nrJ,
+                     const double *nsJ, const double *ntJ, const double *Drq, const double *Dsq,
+                     const double *Dtq, const double *VqLq, const double *VqPq, const double *VfPq,
+                     const double *Jq, double fa, double fb, double fdt, const double *rhsQ_in,
+                     const double *Qf_in, const double *Q_out, double *Q_bar, double *resQ_bar,
+                     const double *rhsQ_bar_out, const double *Qf_bar_out, double *R_bar,
+                     double *sbar, double *rhsQ_bar_in, double *Qf_bar_in, void *stream) {
+  if (!p || K < 0) return WADGES_EINVAL;
+  if (K > 0 && (!vgeo || !vfgeo || !fgeo || !mapPq || !nrJ || !nsJ || !Drq || !Dsq || !VqLq ||
+                !VqPq || !VfPq || !Jq || !rhsQ_in || !Qf_in || !Q_out || !Q_bar || !resQ_bar ||
+                !rhsQ_bar_out || !Qf_bar_out || !R_bar || !sbar || !rhsQ_bar_in || !Qf_bar_in))
+    return WADGES_EINVAL;
+  if (K > 0 && p->dim == 3 && (!ntJ || !Dtq)) return WADGES_EINVAL;
+  return p->ops.stage_vjp(p, K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, Drq, Dsq, Dtq, VqLq,
+                          VqPq, VfPq, Jq, fa, fb, fdt, rhsQ_in, Qf_in, Q_out, Q_bar, resQ_bar,
+                          rhsQ_bar_out, Qf_bar_out, R_bar, sbar, rhsQ_bar_in, Qf_bar_in,
+                          (hipStream_t)stream);
 }
 
 }  // extern "C"

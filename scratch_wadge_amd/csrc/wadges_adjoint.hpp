@@ -329,7 +329,7 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
                   const double *__restrict__ Q, double *__restrict__ Qb,
                   double *__restrict__ resb, const double *__restrict__ uqb,
                   const double *__restrict__ ufb, double *__restrict__ Rb, double gamma,
-                  double lgm1) {
+                  double lgm1, const double *__restrict__ VqLq, double *__restrict__ sbar) {
   constexpr int DIM = S::DIM, NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD;
   constexpr int EPB = 4 * TPW;
   constexpr int MAXF = (NFN + NQ - 1) / NQ;
@@ -463,10 +463,34 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
     for (int f = 0; f < NFLD; ++f) bufB[e][f][i] = r[f] * invJ;
   }
   __syncthreads();
+  double rbar[NFLD];
   if (valid) {
     double r[NFLD];
     mvT(bufB[e], r);
-    for (int f = 0; f < NFLD; ++f) Rb[base + i + f * NQ] = -2.0 * r[f];
+    for (int f = 0; f < NFLD; ++f) {
+      rbar[f] = -2.0 * r[f];
+      Rb[base + i + f * NQ] = rbar[f];
+    }
+  }
+  if (!sbar) return;  // uniform: a kernel argument
+  // s_bar = VqLq^T R_bar at this element's face points (what sbar_kernel computes)
+  __syncthreads();
+  if (valid)
+    for (int f = 0; f < NFLD; ++f) bufA[e][f][i] = rbar[f];
+  __syncthreads();
+  if (valid) {
+    for (int m = 0; m < MAXF; ++m) {
+      const int fn = i + m * NQ;
+      if (fn >= NFN) continue;
+      double v[NFLD];
+      for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
+#pragma unroll 4
+      for (int j = 0; j < NQ; ++j) {
+        const double L = VqLq[j + fn * NQ];
+        for (int f = 0; f < NFLD; ++f) v[f] += L * bufA[e][f][j];
+      }
+      for (int f = 0; f < NFLD; ++f) sbar[basef + f * NFN + fn] = v[f];
+    }
   }
 }
 

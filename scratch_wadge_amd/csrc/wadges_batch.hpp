@@ -353,6 +353,7 @@ int wadges_batch_create(wadges_batch **out, int dim, int ngroups, const wadges_g
       hg[q].a = StageArgs{G.K,    G.vgeo, G.vfgeo, G.fgeo,  G.mapPq, G.nrJ,  G.nsJ,
                           G.ntJ,  ws,     G.VqLq,  G.VqPq,  G.VfPq,  G.Jq,   0.0,
                           0.0,    G.dt,   nullptr, nullptr, G.rhsQ,  G.resQ, G.Q,
+                          G.rhsQ, G.resQ, G.Q,
                           gamma,  gm1,    1.0 / (2.0 * gm1), 0.25 * tau, std::log(gm1)};
       hg[q].qf[0] = G.Qf[0];
       hg[q].qf[1] = G.Qf[1];

@@ -111,9 +111,31 @@ class DeviceSolver:
         for intrk in range(5):
             (self.stage_fused if fused else self.stage)(intrk, dt)
 
-    def run(self, nsteps, dt=None, fused=False):
+    def run(self, nsteps, dt=None, fused=False, graph=False):
+        """nsteps LSRK45 steps.  graph=True replays a HIP graph of two steps
+        (captured through torch.cuda.graph at first use; ten stages bring the
+        double-buffered face traces back to their roles), an odd last step runs
+        directly.  The graph bakes in dt and the state buffers."""
         dt = self.prob.dt if dt is None else dt
-        for _ in range(nsteps):
+        if not graph:
+            for _ in range(nsteps):
+                self.step(dt, fused)
+            return
+        if not hasattr(self, "_graphs"):
+            self._graphs = {}
+        key = (float(dt), bool(fused), self.s["Qf"].data_ptr())   # which face buffer is current
+        if nsteps >= 2 and key not in self._graphs:
+            self.step(dt, fused)              # tables prepared outside the capture
+            self.step(dt, fused)
+            nsteps -= 2
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.step(dt, fused)
+                self.step(dt, fused)
+            self._graphs[key] = g
+        for _ in range(nsteps // 2):
+            self._graphs[key].replay()
+        if nsteps % 2:
             self.step(dt, fused)
 
     def diagnostics(self, Qexact=None):

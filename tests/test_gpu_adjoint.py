@@ -231,3 +231,112 @@ def test_trajectory_matches_reference_gradient(dim, N, steps, ckpt):
     e = relerr(q.grad.cpu().numpy(), qt.grad.cpu().numpy())
     print(f"trajectory gradient vs autograd: {e:.2e}", flush=True)
     assert e < 1e-10
+
+
+# ----------------------------------------------------------------- recorded stages, graphs
+
+def test_stage_record_equals_stage_and_stage_vjp_equals_parts():
+    """wadges_stage_record (out of place) reproduces wadges_stage bit for bit, and
+    wadges_stage_vjp equals wadges_update_vjp followed by wadges_rhs_vjp."""
+    from scratch_wadge_amd.device import _ptr
+    from scratch_wadge_amd.problem import RK4A, RK4B
+    prob = small_problem(3, 3)
+    print("case stage record / stage vjp", flush=True)
+    dev = _solver(prob)
+    c, s = dev.c, dev.s
+    rng = np.random.default_rng(8)
+    dev.set_state(resQ=1e-3 * rng.standard_normal(prob.Q.shape))
+    geo = [_ptr(c[n]) for n in ("vgeo", "vfgeo", "fgeo", "mapPq", "nrJ", "nsJ", "ntJ", "Drq", "Dsq",
+                                "Dtq", "VqLq", "VqPq", "VfPq", "Jq")]
+    ins = {k: v.clone() for k, v in s.items()}
+    outs = {k: torch.empty_like(v) for k, v in s.items()}
+    fa, fb, dt = RK4A[2], RK4B[2], prob.dt
+    dev.plan.stage_record(prob.K, *geo, fa, fb, dt, _ptr(ins["Qf"]), _ptr(outs["Qf"]),
+                          _ptr(ins["rhsQ"]), _ptr(outs["rhsQ"]), _ptr(ins["resQ"]), _ptr(outs["resQ"]),
+                          _ptr(ins["Q"]), _ptr(outs["Q"]), dev.stream())
+    dev.stage_fused(2, dt)
+    torch.cuda.synchronize()
+    for k in ("Q", "resQ", "rhsQ", "Qf"):
+        assert torch.equal(outs[k], s[k]), k
+    # stage VJP against its parts
+    cot = {k: _dev(rng.standard_normal(v.numel())) for k, v in s.items()}
+    Qb1, rb1 = cot["Q"].clone(), cot["resQ"].clone()
+    uqb1, ufb1 = cot["rhsQ"].clone(), cot["Qf"].clone()
+    Rb, sbar = torch.empty_like(Qb1), torch.empty_like(ufb1)
+    dev.plan.stage_vjp(prob.K, *geo, fa, fb, dt, _ptr(ins["rhsQ"]), _ptr(ins["Qf"]), _ptr(outs["Q"]),
+                       _ptr(Qb1), _ptr(rb1), _ptr(uqb1), _ptr(ufb1), _ptr(Rb), _ptr(sbar), _ptr(uqb1),
+                       _ptr(ufb1), dev.stream())
+    Qb2, rb2 = cot["Q"].clone(), cot["resQ"].clone()
+    Rb2, sbar2 = torch.empty_like(Qb2), torch.empty_like(ufb1)
+    uqb2, ufb2 = torch.empty_like(uqb1), torch.empty_like(ufb1)
+    dev.plan.update_vjp(prob.K, _ptr(c["Jq"]), _ptr(c["VqPq"]), _ptr(c["VfPq"]), fa, fb, dt,
+                        _ptr(outs["Q"]), _ptr(Qb2), _ptr(rb2), _ptr(cot["rhsQ"]), _ptr(cot["Qf"]),
+                        _ptr(Rb2), dev.stream())
+    dev.plan.rhs_vjp(prob.K, *geo[:10], _ptr(c["VqLq"]), _ptr(c["VfPq"]), _ptr(ins["rhsQ"]),
+                     _ptr(ins["Qf"]), _ptr(Rb2), _ptr(sbar2), _ptr(uqb2), _ptr(ufb2), dev.stream())
+    torch.cuda.synchronize()
+    for a, b, name in ((Qb1, Qb2, "Q"), (rb1, rb2, "resQ"), (uqb1, uqb2, "rhsQ"), (ufb1, ufb2, "Qf")):
+        e = relerr(a.cpu().numpy(), b.cpu().numpy())
+        print("stage vjp vs parts", name, e, flush=True)
+        assert e < 1e-14, name
+
+
+def test_graph_paths_equal_eager():
+    """Graph-replayed stepping and the graph-replayed adjoint give the results of
+    the eager launches, bit for bit."""
+    from scratch_wadge_amd.adjoint import AdjointSolver
+    prob = small_problem(2, 3)
+    print("case graph vs eager", flush=True)
+    a, b = _solver(prob), _solver(prob)
+    a.run(5, fused=True, graph=True)
+    a.run(3, fused=True, graph=True)
+    b.run(8, fused=True)
+    sa, sb = a.get_state(), b.get_state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    rng = np.random.default_rng(9)
+    Q0 = _dev(prob.Q * (1.0 + 1e-3 * rng.standard_normal(prob.Q.shape)))
+    Qbar = _dev(rng.standard_normal(prob.Q.shape))
+    grads = []
+    for use_graph in (True, False):
+        adj = AdjointSolver(_solver(prob), checkpoint_every=3, use_graph=use_graph)
+        adj.forward(Q0, 7)
+        grads.append(adj.backward(Qbar).cpu().numpy())
+    assert np.array_equal(grads[0], grads[1])
+
+
+def test_misfit_gradient_fd_32_directions():
+    """Density misfit of a 2D vortex (32x32x2 triangles, N=3) after 400 steps
+    against a target grown from Gaussian density bumps of amplitude 1e-2;
+    gradient with respect to the initial conservative state by the adjoint with
+    checkpoints every 40 steps, against central differences on 32 seeded
+    directions."""
+    from scratch_wadge_amd import problem as P
+    from scratch_wadge_amd.adjoint import AdjointSolver, RKRollout, density_misfit, misfit_setup
+    steps = 400
+    prob = P.vortex2d_problem(n=32, N=3)
+    print(f"case misfit gradient K={prob.K} steps={steps}", flush=True)
+    dev = _solver(prob)
+    Q0, _, target = misfit_setup(prob, dev, steps)
+    adj = AdjointSolver(dev, checkpoint_every=40)
+    q = Q0.clone().requires_grad_(True)
+    L = density_misfit(prob, RKRollout.apply(q, adj, steps), target)
+    L.backward()
+    g = q.grad.clone()
+    print(f"loss {float(L.detach()):.6e} |grad| {float(torch.linalg.norm(g)):.6e}", flush=True)
+
+    def loss(Qi):
+        with torch.no_grad():
+            return float(density_misfit(prob, adj.forward(Qi, steps), target))
+
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for k in range(32):
+        d = _dev(rng.standard_normal(prob.Q.shape) * np.abs(prob.Q))
+        eps = 1e-6
+        fd = (loss(Q0 + eps * d) - loss(Q0 - eps * d)) / (2 * eps)
+        a = float(torch.dot(g, d))
+        worst = max(worst, _rel(a, fd))
+        print(f"direction {k}: adjoint {a:.12e} fd {fd:.12e} rel {_rel(a, fd):.2e}", flush=True)
+    print("worst relative difference", worst, flush=True)
+    assert worst < 1e-6
