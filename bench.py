@@ -106,6 +106,20 @@ def cpu_baseline(prob, target_s):
                       f"C++ oracle, OpenMP {threads} threads)"}
 
 
+def profiled_traffic(kernel, workload_file):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE / WRITE_SIZE passes scaled by the copy-kernel calibration), or
+    None when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", workload_file)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return int(parts[2])
+    return None
+
+
 def _sync_time(fn):
     import torch
     torch.cuda.synchronize()
@@ -287,7 +301,12 @@ def main():
             "kernel": "stage_kernel",
             "achieved": round(achieved, 3), "peak": W.FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / W.FP64_PEAK_TFLOPS, 4),
-            "flops_per_launch": flops, "traffic": None,
+            "flops_per_launch": flops,
+            "traffic": profiled_traffic(
+                "stage_kernel", f"stage_kernel_tgv3d_{'affine' if args.affine else 'curved'}"
+                                f"_K{prob.K}_N{prob.N}.md"),
+            "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (copy-calibrated), "
+                              "profiles/stage_kernel_tgv3d_*.md",
         },
         "vol_kernel_roofline": {"achieved": round(vflops / (vol_ms * 1e-3) / 1e12, 3),
                                 "frac": round(vflops / (vol_ms * 1e-3) / 1e12 / W.FP64_PEAK_TFLOPS, 4)},
