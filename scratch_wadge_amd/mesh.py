@@ -45,6 +45,10 @@ class SimplexMesh:
     VX: np.ndarray          # (K, dim+1, dim) unwrapped vertex coordinates per element
     period: np.ndarray      # (dim,) box lengths
     lo: np.ndarray          # (dim,) box lower corner
+    # True: periodic faces do not share vertex ids (meshes read from files); the
+    # face points left unmatched by the vertex ids are matched by coordinates
+    # modulo the box (modify_mapP, euler.c:3004-3175).
+    periodic_points: bool = False
 
 
 def periodic_box_tri(n, lo=-5.0, hi=5.0, perturb=0.0, seed=0):
@@ -258,11 +262,13 @@ def build_geometry(mesh, ops: RefOps, warp=None) -> Geometry:
 FACE_VERTS = {2: [[0, 1], [1, 2], [0, 2]], 3: [[0, 1, 2], [0, 1, 3], [1, 2, 3], [0, 2, 3]]}
 
 
-def build_mapPq(mesh, ops: RefOps, nfields=None):
+def build_mapPq(mesh, ops: RefOps, nfields=None, xf=None):
     """Field-strided neighbour map mapPq[k][Nf] (euler.c:3158-3170).
 
-    Unmatched face points (none on the periodic meshes) map to themselves,
-    which the surface kernel treats as a zero jump."""
+    Face points are matched through the shared vertex ids and their barycentric
+    weights; for meshes with mesh.periodic_points the points left over are then
+    matched by their coordinates xf (dim, Nf, K) modulo the box.  Unmatched face
+    points map to themselves, which the surface kernel treats as a zero jump."""
     dim = mesh.dim
     nfields = nfields or dim + 2
     Nfq, Nfaces = ops.Nfq, ops.Nfaces
@@ -298,6 +304,135 @@ def build_mapPq(mesh, ops: RefOps, nfields=None):
     mapP = mapP.copy()
     mapP[a] = node[b] + elem[b] * Nf
     mapP[b] = node[a] + elem[a] * Nf
+    nmatched = int(np.sum(same))
+    if mesh.periodic_points:
+        if xf is None:
+            raise ValueError("periodic matching by coordinates needs the face points xf")
+        X = np.transpose(xf, (0, 2, 1)).reshape(dim, -1)          # index k*Nf + node
+        nmatched += _match_periodic_points(mapP, X, mesh.lo, mesh.period)
     en = mapP // Nf
     mapPq = (mapP - en * Nf) + en * Nf * nfields
-    return np.ascontiguousarray(mapPq.reshape(K, Nf).astype(np.uint32)), int(np.sum(same))
+    return np.ascontiguousarray(mapPq.reshape(K, Nf).astype(np.uint32)), nmatched
+
+
+# ------------------------------------------------------------------ gmsh ingest and connectivity
+
+# Face-to-vertex and orientation tables of the reference's connectivity
+# (euler.c:429-510): face f of an element has vertices FToFV[f]; a face seen
+# from the neighbour is the same vertex list permuted by OToFV[o].
+FTOFV = {2: np.array([[0, 1], [1, 2], [2, 0]]),
+         3: np.array([[0, 1, 2], [0, 1, 3], [1, 2, 3], [0, 2, 3]])}
+OTOFV = {2: np.array([[0, 1], [1, 0]]),
+         3: np.array([[0, 1, 2], [2, 0, 1], [1, 2, 0], [1, 0, 2], [2, 1, 0], [0, 2, 1]])}
+GMSH_TYPE = {2: 2, 3: 4}     # gmsh element type of a triangle / tetrahedron
+
+
+def read_gmsh(path, dim):
+    """gmsh 2.2 ASCII reader in the manner of host_mesh_read_msh (euler.c:556-701):
+    $Nodes (id x y z), $Elements of the triangle (2D) or tetrahedron (3D) type
+    with their tags skipped; other element types and the $Periodic section are
+    ignored.  Returns (VX (Nv, dim) vertex coordinates, EToV (K, dim+1)
+    zero-based vertex ids in file order)."""
+    want = GMSH_TYPE[dim]
+    nodes, elems = {}, []
+    section = None
+    with open(path) as f:
+        lines = iter(f.read().splitlines())
+        for line in lines:
+            s = line.strip()
+            if s.startswith("$"):
+                section = s[1:] if not s.startswith("$End") else None
+                if section in ("Nodes", "Elements"):
+                    next(lines)                      # count line
+                continue
+            if not s:
+                continue
+            if section == "Nodes":
+                w = s.split()
+                nodes[int(w[0])] = [float(x) for x in w[1:1 + dim]]
+            elif section == "Elements":
+                w = [int(x) for x in s.split()]
+                if w[1] == want:
+                    ntags = w[2]
+                    elems.append(w[3 + ntags:3 + ntags + dim + 1])
+    if not nodes or not elems:
+        raise ValueError(f"{path}: no $Nodes or no dimension-{dim} simplices")
+    ids = np.array(sorted(nodes))
+    VX = np.array([nodes[i] for i in ids], dtype=np.float64)
+    remap = np.full(ids.max() + 1, -1, dtype=np.int64)
+    remap[ids] = np.arange(len(ids))
+    EToV = remap[np.array(elems, dtype=np.int64)]
+    if (EToV < 0).any():
+        raise ValueError(f"{path}: element refers to a missing node")
+    return VX, EToV
+
+
+def connect(EToV, dim):
+    """Face connectivity with the reference's conventions (host_mesh_connect,
+    euler.c:939-1612, single rank): faces are matched through their sorted vertex
+    ids; EToE[e][f], EToF[e][f] name the neighbour element and face, EToO[e][f] the
+    orientation o with EToV[e][FToFV[f][OToFV[o][v]]] == EToV[e1][FToFV[f1][v]]
+    (the relation euler.c:1663-1705 checks).  Unmatched faces point to
+    themselves with orientation 0 (euler.c:1565-1573)."""
+    K = EToV.shape[0]
+    nfc = dim + 1
+    fv = EToV[:, FTOFV[dim]]                                 # (K, nfc, nfv)
+    keys = np.sort(fv, axis=2).reshape(K * nfc, -1)
+    srt = np.lexsort(keys.T[::-1])
+    ks = keys[srt]
+    same = np.all(ks[1:] == ks[:-1], axis=1)
+    if np.any(same[1:] & same[:-1]):
+        raise ValueError("face shared by more than two elements")
+    EToE = np.repeat(np.arange(K)[:, None], nfc, 1)
+    EToF = np.repeat(np.arange(nfc)[None, :], K, 0)
+    EToO = np.zeros((K, nfc), dtype=np.int64)
+    a, b = srt[:-1][same], srt[1:][same]
+    ea, fa, eb, fb = a // nfc, a % nfc, b // nfc, b % nfc
+    EToE[ea, fa], EToF[ea, fa] = eb, fb
+    EToE[eb, fb], EToF[eb, fb] = ea, fa
+    perms = OTOFV[dim]
+    for (e0, f0, e1, f1) in ((ea, fa, eb, fb), (eb, fb, ea, fa)):
+        v0, v1 = fv[e0, f0], fv[e1, f1]                      # (m, nfv)
+        hit = np.all(v0[:, perms] == v1[:, None, :], axis=2)  # (m, norien)
+        if not hit.any(1).all():
+            raise ValueError("matched faces with no orientation relation")
+        EToO[e0, f0] = hit.argmax(1)
+    return EToE, EToF, EToO
+
+
+def _match_periodic_points(mapP, X, lo, period, tol=1e-7):
+    """Pairs the face points that map to themselves by their coordinates modulo
+    the box (modify_mapP, euler.c:3004-3175, which compares all pairs; here the
+    wrapped coordinates are hashed).  Updates mapP in place; returns the number
+    of pairs."""
+    free = np.nonzero(mapP == np.arange(mapP.size))[0]
+    if free.size == 0:
+        return 0
+    Y = (X[:, free].T - lo) / period                       # (m, dim) in box units
+    Y = Y - np.floor(Y)
+    Y[Y > 1.0 - tol] -= 1.0                                # the upper face wraps to the lower
+    q = np.round(Y / (10 * tol)).astype(np.int64)
+    srt = np.lexsort(q.T[::-1])
+    qs = q[srt]
+    same = np.all(qs[1:] == qs[:-1], axis=1)
+    if np.any(same[1:] & same[:-1]):
+        raise ValueError("more than two face points coincide modulo the box")
+    a, b = free[srt[:-1][same]], free[srt[1:][same]]
+    d = X[:, a] - X[:, b]
+    d = d - np.round(d / period[:, None]) * period[:, None]
+    if np.abs(d).max(initial=0.0) > 1e3 * tol * period.max():
+        raise ValueError("periodic face points do not coincide")
+    mapP[a], mapP[b] = b, a
+    return int(a.size)
+
+
+def mesh_from_gmsh(path, dim, periodic=True):
+    """SimplexMesh from a gmsh 2.2 file.  Interior faces are matched through the
+    file's vertex ids; with periodic=True the faces on opposite sides of the
+    bounding box are matched point by point by coordinates (like modify_mapP),
+    not through vertex ids, which coarse periodic meshes make ambiguous."""
+    VX, EToV = read_gmsh(path, dim)
+    lo, period = VX.min(0), VX.max(0) - VX.min(0)
+    m = SimplexMesh(dim, EToV.shape[0], EToV.copy(), VX[EToV].copy(), period, lo,
+                    periodic_points=periodic)
+    return _orient(m)

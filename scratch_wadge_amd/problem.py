@@ -179,7 +179,7 @@ def make_problem(dim, N, mesh, ic, warp=None, gamma=GAMMA, tau=1.0, cfl=0.5, sta
     """Assemble a Problem.  ic(xq) -> (nfields, Nq, K) conservative state at quadrature points."""
     ops = build_ref_ops(dim, N)
     geo = meshmod.build_geometry(mesh, ops, warp=warp)
-    mapPq, nmatched = meshmod.build_mapPq(mesh, ops)
+    mapPq, nmatched = meshmod.build_mapPq(mesh, ops, xf=geo.xf)
     Uq = ic(*geo.xq) if state is None else state
     Q, Uvq, Uvf = project_initial_state(ops, Uq, gamma)
     dt, hmin = time_step(dim, N, geo.Jq, geo.fgeo, cfl)
