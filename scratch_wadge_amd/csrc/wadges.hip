@@ -102,6 +102,15 @@ template <class S> __device__ __forceinline__ int vf_partner(int t, int i) {
 
 template <class S> constexpr int vf_steps() { return face_chunked<S>() ? S::NFN : S::L; }
 
+// Face-chunked tetrahedra whose lanes form at most two groups visit the faces in
+// two halves (faces 0-1, then 2-3), the two groups on different faces of the
+// half at every step; only the current half's face records are held in LDS,
+// which raises occupancy (3D N = 3: 4 instead of 2.75 waves per SIMD).  The
+// steps t and their partners are those of vf_partner, taken in another order.
+template <class S> constexpr int face_halves() {
+  return (face_chunked<S>() && S::NFACES == 4 && (S::NQ + S::NFQ - 1) / S::NFQ <= 2) ? 2 : 1;
+}
+
 // ------------------------------------------------------------------ operator prep
 
 template <class S>
@@ -137,10 +146,13 @@ __global__ void prep_vol_tables(const double *__restrict__ Drq, const double *__
 // volume-node records (volume-volume block) and then for the face-node records
 // and face accumulator (volume-face block).
 template <class S> struct VolSmem {
+  static constexpr int H = face_halves<S>();
+  static constexpr int NH = S::NFN / H;                    // face records held at a time
   static constexpr int acc_v = S::NFLD * S::NQ;
   static constexpr int phase1 = S::VREC * S::NQ;
-  static constexpr int phase2 = (S::FREC + S::NFLD) * S::NFN;
+  static constexpr int phase2 = S::FREC * NH + S::NFLD * S::NFN;
   static constexpr int per_elem = acc_v + (phase1 > phase2 ? phase1 : phase2);
+  static constexpr int accF_off = acc_v + S::FREC * NH;    // face accumulators [NFLD][NFN]
 };
 
 // Waves per SIMD that the LDS of a one-wave workgroup of `bytes` allows (160 KiB
@@ -196,9 +208,10 @@ __device__ __forceinline__ void vol_body(bool valid, int i, const double *Uq, co
   constexpr int DIM = S::DIM, NQ = S::NQ, NFQ = S::NFQ, NFN = S::NFN, NFLD = S::NFLD;
   constexpr int NG = S::NG, TVT = S::TVT, VS = S::VS, FREC = S::FREC;
   constexpr int T = vf_steps<S>();
+  constexpr int H = VolSmem<S>::H, NH = VolSmem<S>::NH;
   double *vrec = accV + NFLD * NQ;         // phase 1
-  double *frec = vrec;                      // phase 2
-  double *accF = frec + FREC * NFN;
+  double *frec = vrec;                      // phase 2: records of NH face points
+  double *accF = accV + VolSmem<S>::accF_off;
 
   // ---------------- phase 1: volume records, volume-volume block
   Node<DIM> me;
@@ -250,71 +263,83 @@ __device__ __forceinline__ void vol_body(bool valid, int i, const double *Uq, co
   }
   __syncthreads();
 
-  // ---------------- phase 2: face records (overwrite the volume records)
-  if (valid) {
-    for (int fn = i; fn < NFN; fn += NQ) {
-      double U[NFLD];
-      for (int f = 0; f < NFLD; ++f) U[f] = Uf[fn + f * NFN];
-      store_node<DIM>(frec, NFN, fn, make_node<DIM>(U, gm1));
-      const int face = fn / NFQ;
-      for (int d = 0; d < DIM; ++d) {
-        double b = 0.0;
-        for (int r = 0; r < DIM; ++r) b = b + 0.5 * vfg[fn + (r * DIM + d) * NFN] * s_nref[face][r];
-        frec[(VS + d) * NFN + fn] = b;
-      }
+  // ---------------- phase 2: face records (overwrite the volume records),
+  // volume-face block, in H parts of NH face points
+  if (valid)
+    for (int fn = i; fn < NFN; fn += NQ)
       for (int f = 0; f < NFLD; ++f) accF[f * NFN + fn] = 0.0;
+  const double *W1 = ws + S::WS_D;
+  const double *W2 = W1 + T * NQ;
+  // one volume-face pair: step t, partner face point fn (record fl), own-side normal part a
+  auto vf_pair = [&](int t, int fn, int fl, const double (&a)[DIM]) {
+    const Node<DIM> pf = load_node<DIM>(frec, NH, fl);
+    double n[DIM];
+    for (int d = 0; d < DIM; ++d) n[d] = a[d] + frec[(VS + d) * NH + fl];
+    const PairFlux<DIM> P = pair_flux<DIM>(me, pf, inv2gm1);
+    double Fn[NFLD];
+    flux_dot<DIM>(P, n, Fn);
+    const double w1 = W1[t * NQ + i];
+    const double w2 = W2[t * NQ + i];
+    for (int f = 0; f < NFLD; ++f) {
+      acc[f] += w1 * Fn[f];
+      lds_add(&accF[f * NFN + fn], w2 * Fn[f]);
     }
-  }
-  __syncthreads();
-
-  if (valid) {
-    const double *W1 = ws + S::WS_D;
-    const double *W2 = W1 + T * NQ;
-    // one volume-face pair: partner face point fn, own-side normal part a
-    auto vf_pair = [&](int t, int fn, const double (&a)[DIM]) {
-      const Node<DIM> pf = load_node<DIM>(frec, NFN, fn);
-      double n[DIM];
-      for (int d = 0; d < DIM; ++d) n[d] = a[d] + frec[(VS + d) * NFN + fn];
-      const PairFlux<DIM> P = pair_flux<DIM>(me, pf, inv2gm1);
-      double Fn[NFLD];
-      flux_dot<DIM>(P, n, Fn);
-      const double w1 = W1[t * NQ + i];
-      const double w2 = W2[t * NQ + i];
-      for (int f = 0; f < NFLD; ++f) {
-        acc[f] += w1 * Fn[f];
-        lds_add(&accF[f * NFN + fn], w2 * Fn[f]);
-      }
-    };
-    // own-side part of the averaged normal on a face: sum_r (G_rd / 2) nref_r
-    auto own_normal = [&](int face, double (&a)[DIM]) {
-      for (int d = 0; d < DIM; ++d) {
-        double v = 0.0;
-        for (int r = 0; r < DIM; ++r) v = v + hG[r * DIM + d] * s_nref[face][r];
-        a[d] = v;
-      }
-    };
-    if constexpr (face_chunked<S>()) {
-      const int g = i / NFQ, o = i - (i / NFQ) * NFQ;
-      for (int c = 0; c < S::NFACES; ++c) {
-        int face = c + g;
-        if (face >= S::NFACES) face -= S::NFACES;
-        double a[DIM];
-        own_normal(face, a);
-        for (int m = 0; m < NFQ; ++m) {
-          int node = o + m;
-          if (node >= NFQ) node -= NFQ;
-          vf_pair(c * NFQ + m, face * NFQ + node, a);
+  };
+  // own-side part of the averaged normal on a face: sum_r (G_rd / 2) nref_r
+  auto own_normal = [&](int face, double (&a)[DIM]) {
+    for (int d = 0; d < DIM; ++d) {
+      double v = 0.0;
+      for (int r = 0; r < DIM; ++r) v = v + hG[r * DIM + d] * s_nref[face][r];
+      a[d] = v;
+    }
+  };
+  for (int h = 0; h < H; ++h) {
+    if (valid) {
+      for (int fl = i; fl < NH; fl += NQ) {
+        const int fn = h * NH + fl;
+        double U[NFLD];
+        for (int f = 0; f < NFLD; ++f) U[f] = Uf[fn + f * NFN];
+        store_node<DIM>(frec, NH, fl, make_node<DIM>(U, gm1));
+        const int face = fn / NFQ;
+        for (int d = 0; d < DIM; ++d) {
+          double b = 0.0;
+          for (int r = 0; r < DIM; ++r) b = b + 0.5 * vfg[fn + (r * DIM + d) * NFN] * s_nref[face][r];
+          frec[(VS + d) * NH + fl] = b;
         }
       }
-    } else {
-      for (int t = 0; t < T; ++t) {
-        const int fn = vf_partner<S>(t, i);
-        if (fn >= NFN) continue;
-        double a[DIM];
-        own_normal(fn / NFQ, a);
-        vf_pair(t, fn, a);
+    }
+    __syncthreads();
+    if (valid) {
+      if constexpr (face_chunked<S>()) {
+        const int g = i / NFQ, o = i - (i / NFQ) * NFQ;
+        constexpr int FH = S::NFACES / H;            // faces per part
+        for (int u = 0; u < FH; ++u) {
+          // face of this group at part-step u; the chunk c of vf_partner that visits it
+          int fh = u + g;
+          while (fh >= FH) fh -= FH;
+          const int face = h * FH + fh;
+          int c = face - g;
+          if (c < 0) c += S::NFACES;
+          double a[DIM];
+          own_normal(face, a);
+          for (int m = 0; m < NFQ; ++m) {
+            int node = o + m;
+            if (node >= NFQ) node -= NFQ;
+            const int fn = face * NFQ + node;
+            vf_pair(c * NFQ + m, fn, fn - h * NH, a);
+          }
+        }
+      } else {
+        for (int t = 0; t < T; ++t) {
+          const int fn = vf_partner<S>(t, i);
+          if (fn >= NFN) continue;
+          double a[DIM];
+          own_normal(fn / NFQ, a);
+          vf_pair(t, fn, fn, a);
+        }
       }
     }
+    if (H > 1) __syncthreads();   // records are overwritten by the next part
   }
   __syncthreads();
 
@@ -355,7 +380,7 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
   double acc[NFLD];
   vol_body<S>(valid, i, rhsQ + kk * NQ * NFLD, Qf + kk * NFN * NFLD, vgeo + kk * NQ * NG,
               vfgeo + kk * NFN * NG, s_nref, ws, accV, gm1, inv2gm1, acc);
-  const double *accF = accV + NFLD * NQ + FREC * NFN;
+  const double *accF = accV + VolSmem<S>::accF_off;
 
   if (valid) {
     double *dst = rhsQ + (size_t)k * NQ * NFLD;
@@ -535,14 +560,10 @@ update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ V
       conservative_vars<DIM>(V, gamma, lgm1, U);
       for (int f = 0; f < NFLD; ++f) Qf[basef + fn + f * NFN] = U[f];
     }
-    // volume points: rhsQ = u(VqPq v~)
+    // volume points: rhsQ = u(VqPq v~) = u(v~) (VqPq is a projection and v~
+    // already lies in its range, round-off apart)
     double V[NFLD], U[NFLD];
-    for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
-#pragma unroll 2
-    for (int j = 0; j < NQ; ++j) {
-      const double P = sP[i + j * NQ];
-      for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[e][f][j];
-    }
+    for (int f = 0; f < NFLD; ++f) V[f] = bufA[e][f][i];
     conservative_vars<DIM>(V, gamma, lgm1, U);
     for (int f = 0; f < NFLD; ++f) rhsQ[base + i + f * NQ] = U[f];
   }
@@ -603,38 +624,44 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
 
   load_nref<S>(lane, A->nrJ, A->nsJ, A->ntJ, s_nref);
   double *accV = smem + (active ? team : 0) * PER;
-  double *accF = accV + NFLD * NQ + FREC * NFN;
+  double *accF = accV + VolSmem<S>::accF_off;
   double acc[NFLD];
   vol_body<S>(valid, i, A->rhsQ_in + base, Qf_in + kk * NFN * NFLD, A->vgeo + kk * NQ * NG,
               A->vfgeo + kk * NFN * NG, s_nref, A->ws, accV, gm1, inv2gm1, acc);
 
-  // ---------------- surface flux + LF penalty, in place over the face rows
-  if (valid) {
-    const double *g = A->fgeo + kk * NFN * NFG;
-    for (int fn = i; fn < NFN; fn += NQ) {
-      const size_t idM = (size_t)fn + kk * NFN * NFLD;
-      const size_t idP = A->mapPq[kk * NFN + fn];
-      double UM[NFLD], UP[NFLD];
-      for (int f = 0; f < NFLD; ++f) {
-        UM[f] = Qf_in[idM + (size_t)f * NFN];
-        UP[f] = Qf_in[idP + (size_t)f * NFN];
-      }
-      double nJ[DIM];
-      for (int d = 0; d < DIM; ++d) nJ[d] = g[fn + d * NFN];
-      const double sJ = g[fn + DIM * NFN];
-      const Node<DIM> a = make_node<DIM>(UM, gm1);
-      const Node<DIM> b = make_node<DIM>(UP, gm1);
-      const double pM = gm1 * (UM[NFLD - 1] - 0.5 * UM[0] * (2.0 * a.hq));
-      const double pP = gm1 * (UP[NFLD - 1] - 0.5 * UP[0] * (2.0 * b.hq));
-      const double lamM = sqrt(2.0 * a.hq) + sqrt(gamma * pM / UM[0]);
-      const double lamP = sqrt(2.0 * b.hq) + sqrt(gamma * pP / UP[0]);
-      const double Lfc = ltau * fmax(lamM, lamP) * sJ;
-      const PairFlux<DIM> P = pair_flux<DIM>(a, b, inv2gm1);
-      double Fn[NFLD];
-      flux_dot<DIM>(P, nJ, Fn);
-      for (int f = 0; f < NFLD; ++f)
-        accF[f * NFN + fn] = 0.5 * Fn[f] + accF[f * NFN + fn] - Lfc * (UP[f] - UM[f]);
+  // ---------------- surface flux + LF penalty, in place over the face rows.
+  // The face points of all TPW elements are spread over the whole wave (2
+  // passes instead of 3 at Nq = 23, Nf = 48).
+  for (int p = lane; p < TPW * NFN; p += WAVE) {
+    const int t = p / NFN;
+    const int fn = p - t * NFN;
+    const int kt = kfirst + t;
+    if (kt >= A->K) continue;
+    const size_t kq = (size_t)kt;
+    double *accFt = smem + t * PER + VolSmem<S>::accF_off;
+    const double *g = A->fgeo + kq * NFN * NFG;
+    const size_t idM = (size_t)fn + kq * NFN * NFLD;
+    const size_t idP = A->mapPq[kq * NFN + fn];
+    double UM[NFLD], UP[NFLD];
+    for (int f = 0; f < NFLD; ++f) {
+      UM[f] = Qf_in[idM + (size_t)f * NFN];
+      UP[f] = Qf_in[idP + (size_t)f * NFN];
     }
+    double nJ[DIM];
+    for (int d = 0; d < DIM; ++d) nJ[d] = g[fn + d * NFN];
+    const double sJ = g[fn + DIM * NFN];
+    const Node<DIM> a = make_node<DIM>(UM, gm1);
+    const Node<DIM> b = make_node<DIM>(UP, gm1);
+    const double pM = gm1 * (UM[NFLD - 1] - 0.5 * UM[0] * (2.0 * a.hq));
+    const double pP = gm1 * (UP[NFLD - 1] - 0.5 * UP[0] * (2.0 * b.hq));
+    const double lamM = sqrt(2.0 * a.hq) + sqrt(gamma * pM / UM[0]);
+    const double lamP = sqrt(2.0 * b.hq) + sqrt(gamma * pP / UP[0]);
+    const double Lfc = ltau * fmax(lamM, lamP) * sJ;
+    const PairFlux<DIM> P = pair_flux<DIM>(a, b, inv2gm1);
+    double Fn[NFLD];
+    flux_dot<DIM>(P, nJ, Fn);
+    for (int f = 0; f < NFLD; ++f)
+      accFt[f * NFN + fn] = 0.5 * Fn[f] + accFt[f * NFN + fn] - Lfc * (UP[f] - UM[f]);
   }
   __syncthreads();
   // ---------------- RHS row i = volume part + lift of the face terms
@@ -701,21 +728,29 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
     for (int f = 0; f < NFLD; ++f) bufA[f * NQ + i] = r[f];
   }
   __syncthreads();
-  if (valid) {
-    double *qf = Qf_out + kk * NFN * NFLD;
-    for (int fn = i; fn < NFN; fn += NQ) {
-      double V[NFLD], U[NFLD];
-      for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
-#pragma unroll 8
-      for (int j = 0; j < NQ; ++j) {
-        const double P = A->VfPq[fn + j * NFN];
-        for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[f * NQ + j];
-      }
-      conservative_vars<DIM>(V, gamma, lgm1, U);
-      for (int f = 0; f < NFLD; ++f) qf[fn + f * NFN] = U[f];
-    }
+  // face points, spread over the whole wave: Qf = u(VfPq v~)
+  for (int p = lane; p < TPW * NFN; p += WAVE) {
+    const int t = p / NFN;
+    const int fn = p - t * NFN;
+    const int kt = kfirst + t;
+    if (kt >= A->K) continue;
+    const double *bufAt = smem + t * PER;
     double V[NFLD], U[NFLD];
-    matvec(bufA, 1.0, V);
+    for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
+#pragma unroll 8
+    for (int j = 0; j < NQ; ++j) {
+      const double P = A->VfPq[fn + j * NFN];
+      for (int f = 0; f < NFLD; ++f) V[f] += P * bufAt[f * NQ + j];
+    }
+    conservative_vars<DIM>(V, gamma, lgm1, U);
+    double *qf = Qf_out + (size_t)kt * NFN * NFLD;
+    for (int f = 0; f < NFLD; ++f) qf[fn + f * NFN] = U[f];
+  }
+  // volume points: rhsQ = u(VqPq v~) = u(v~), since VqPq is a projection and
+  // v~ = VqPq (...) already lies in its range (round-off apart)
+  if (valid) {
+    double V[NFLD], U[NFLD];
+    for (int f = 0; f < NFLD; ++f) V[f] = bufA[f * NQ + i];
     conservative_vars<DIM>(V, gamma, lgm1, U);
     for (int f = 0; f < NFLD; ++f) A->rhsQ[base + i + f * NQ] = U[f];
   }
