@@ -149,3 +149,18 @@ def test_geometric_conservation_law():
             c = np.einsum("mi,ki->km", ops.Pq, G[:, r * 3 + d, :])
             div[:, d, :] += c @ ops.Dmodal[r].T
     assert np.abs(div).max() < 1e-12 * np.abs(G).max()
+
+
+def test_vortex_ensemble_generator():
+    """The ensemble workload: K log-uniform in [128, 8192] rounded to 2 n^2,
+    N in {2, 3, 4}, reproducible from the seed."""
+    import numpy as np
+    from scratch_wadge_amd.ensemble import vortex_ensemble
+    a = vortex_ensemble(12, seed=5)
+    b = vortex_ensemble(12, seed=5)
+    assert [(p.K, p.N) for p in a] == [(p.K, p.N) for p in b]
+    for p in a:
+        n = int(round(np.sqrt(p.K / 2)))
+        assert p.K == 2 * n * n and 64 <= p.K <= 2 * 64 * 64
+        assert p.N in (2, 3, 4) and p.dim == 2 and p.dt > 0
+    assert np.array_equal(a[0].Q, b[0].Q)
