@@ -40,6 +40,22 @@ def vortex_ensemble(count=64, seed=0, kmin=128, kmax=8192, degrees=(2, 3, 4), st
     return out
 
 
+def group_of(sv: DeviceSolver):
+    """wadges_group describing one DeviceSolver's arrays (Qf[0] = its current
+    face traces, Qf[1] = its second buffer)."""
+    p, c, s = sv.prob, sv.c, sv.s
+    g = _lib.Group()
+    g.N, g.Nq, g.Nfq, g.K, g.dt = p.N, p.Nq, p.Nfq, p.K, p.dt
+    for name in ("vgeo", "vfgeo", "fgeo", "mapPq", "nrJ", "nsJ", "ntJ", "Drq", "Dsq", "Dtq",
+                 "VqLq", "VqPq", "VfPq", "Jq"):
+        setattr(g, name, c[name].data_ptr() if c.get(name) is not None else None)
+    for name in ("Q", "rhsQ", "resQ"):
+        setattr(g, name, s[name].data_ptr())
+    g.Qf[0] = s["Qf"].data_ptr()
+    g.Qf[1] = sv.Qf_alt.data_ptr()
+    return g
+
+
 class Ensemble:
     def __init__(self, problems, device="cuda:0"):
         if not problems:
@@ -55,19 +71,7 @@ class Ensemble:
         self.problems = problems
         self.solvers = [DeviceSolver(p, device) for p in problems]
         self.device = self.solvers[0].device
-        groups = []
-        for sv in self.solvers:
-            p, c, s = sv.prob, sv.c, sv.s
-            g = _lib.Group()
-            g.N, g.Nq, g.Nfq, g.K, g.dt = p.N, p.Nq, p.Nfq, p.K, p.dt
-            for name in ("vgeo", "vfgeo", "fgeo", "mapPq", "nrJ", "nsJ", "ntJ", "Drq", "Dsq", "Dtq",
-                         "VqLq", "VqPq", "VfPq", "Jq"):
-                setattr(g, name, c[name].data_ptr() if c.get(name) is not None else None)
-            for name in ("Q", "rhsQ", "resQ"):
-                setattr(g, name, s[name].data_ptr())
-            g.Qf[0] = s["Qf"].data_ptr()
-            g.Qf[1] = sv.Qf_alt.data_ptr()
-            groups.append(g)
+        groups = [group_of(sv) for sv in self.solvers]
         self._qf = [(sv.s["Qf"], sv.Qf_alt) for sv in self.solvers]   # fixed buffer roles
         self.batch = _lib.Batch(self.dim, groups, problems[0].gamma, problems[0].tau, self.stream())
         self.K_total = sum(p.K for p in problems)
