@@ -195,16 +195,25 @@ __device__ __forceinline__ void store_node(double *rec, int stride, int j, const
   rec[(DIM + 4) * stride + j] = n.hq;
 }
 
+// Where the other teams of the wave find their data (for phases that spread the
+// face points of all elements of a wave over all 64 lanes).
+struct WaveCtx {
+  int lane, kfirst, K;
+  const double *Uf;    // face state of element 0 of the mesh
+  const double *vfg;   // face geofacs of element 0
+  double *smem;        // the workgroup's dynamic LDS (team t at t * per_elem)
+};
+
 // The volume term of one element (both coupling blocks).  On return acc holds
 // the register part of volume row i, accV the LDS part (partner contributions),
-// accF the face rows (rhsQf).  Uq is the element's projected volume state,
-// Uf its face state.  Every lane of the workgroup must call this (barriers).
+// accF the face rows (rhsQf).  Uq is the element's projected volume state, vg
+// its volume geofacs; the face state and geofacs of the wave's elements come
+// through wc.  Every lane of the workgroup must call this (barriers).
 template <class S>
-__device__ __forceinline__ void vol_body(bool valid, int i, const double *Uq, const double *Uf,
-                                         const double *vg, const double *vfg,
+__device__ __forceinline__ void vol_body(bool valid, int i, const double *Uq, const double *vg,
                                          const double (*s_nref)[S::DIM], const double *ws,
                                          double *accV, double gm1, double inv2gm1,
-                                         double (&acc)[S::NFLD]) {
+                                         double (&acc)[S::NFLD], const WaveCtx &wc) {
   constexpr int DIM = S::DIM, NQ = S::NQ, NFQ = S::NFQ, NFN = S::NFN, NFLD = S::NFLD;
   constexpr int NG = S::NG, TVT = S::TVT, VS = S::VS, FREC = S::FREC;
   constexpr int T = vf_steps<S>();
@@ -264,10 +273,13 @@ __device__ __forceinline__ void vol_body(bool valid, int i, const double *Uq, co
   __syncthreads();
 
   // ---------------- phase 2: face records (overwrite the volume records),
-  // volume-face block, in H parts of NH face points
-  if (valid)
-    for (int fn = i; fn < NFN; fn += NQ)
-      for (int f = 0; f < NFLD; ++f) accF[f * NFN + fn] = 0.0;
+  // volume-face block, in H parts of NH face points.  Records and accumulator
+  // clearing are spread over the whole wave (all teams' face points together).
+  constexpr int TPW = WAVE / NQ, PER = VolSmem<S>::per_elem;
+  for (int p = wc.lane; p < TPW * NFN * NFLD; p += WAVE) {
+    const int t = p / (NFN * NFLD);
+    wc.smem[t * PER + VolSmem<S>::accF_off + (p - t * NFN * NFLD)] = 0.0;
+  }
   const double *W1 = ws + S::WS_D;
   const double *W2 = W1 + T * NQ;
   // one volume-face pair: step t, partner face point fn (record fl), own-side normal part a
@@ -294,18 +306,23 @@ __device__ __forceinline__ void vol_body(bool valid, int i, const double *Uq, co
     }
   };
   for (int h = 0; h < H; ++h) {
-    if (valid) {
-      for (int fl = i; fl < NH; fl += NQ) {
-        const int fn = h * NH + fl;
-        double U[NFLD];
-        for (int f = 0; f < NFLD; ++f) U[f] = Uf[fn + f * NFN];
-        store_node<DIM>(frec, NH, fl, make_node<DIM>(U, gm1));
-        const int face = fn / NFQ;
-        for (int d = 0; d < DIM; ++d) {
-          double b = 0.0;
-          for (int r = 0; r < DIM; ++r) b = b + 0.5 * vfg[fn + (r * DIM + d) * NFN] * s_nref[face][r];
-          frec[(VS + d) * NH + fl] = b;
-        }
+    for (int p = wc.lane; p < TPW * NH; p += WAVE) {
+      const int t = p / NH;
+      const int fl = p - t * NH;
+      const int kt = wc.kfirst + t;
+      if (kt >= wc.K) continue;
+      const double *Uft = wc.Uf + (size_t)kt * NFN * NFLD;
+      const double *vfgt = wc.vfg + (size_t)kt * NFN * NG;
+      double *frect = wc.smem + t * PER + NFLD * NQ;
+      const int fn = h * NH + fl;
+      double U[NFLD];
+      for (int f = 0; f < NFLD; ++f) U[f] = Uft[fn + f * NFN];
+      store_node<DIM>(frect, NH, fl, make_node<DIM>(U, gm1));
+      const int face = fn / NFQ;
+      for (int d = 0; d < DIM; ++d) {
+        double b = 0.0;
+        for (int r = 0; r < DIM; ++r) b = b + 0.5 * vfgt[fn + (r * DIM + d) * NFN] * s_nref[face][r];
+        frect[(VS + d) * NH + fl] = b;
       }
     }
     __syncthreads();
@@ -378,8 +395,9 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
   load_nref<S>(lane, nrJ, nsJ, ntJ, s_nref);
   double *accV = smem + (active ? team : 0) * PER;
   double acc[NFLD];
-  vol_body<S>(valid, i, rhsQ + kk * NQ * NFLD, Qf + kk * NFN * NFLD, vgeo + kk * NQ * NG,
-              vfgeo + kk * NFN * NG, s_nref, ws, accV, gm1, inv2gm1, acc);
+  const WaveCtx wc{lane, (int)blockIdx.x * TPW, K, Qf, vfgeo, smem};
+  vol_body<S>(valid, i, rhsQ + kk * NQ * NFLD, vgeo + kk * NQ * NG, s_nref, ws, accV, gm1,
+              inv2gm1, acc, wc);
   const double *accF = accV + VolSmem<S>::accF_off;
 
   if (valid) {
@@ -626,8 +644,9 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   double *accV = smem + (active ? team : 0) * PER;
   double *accF = accV + VolSmem<S>::accF_off;
   double acc[NFLD];
-  vol_body<S>(valid, i, A->rhsQ_in + base, Qf_in + kk * NFN * NFLD, A->vgeo + kk * NQ * NG,
-              A->vfgeo + kk * NFN * NG, s_nref, A->ws, accV, gm1, inv2gm1, acc);
+  const WaveCtx wc{lane, kfirst, A->K, Qf_in, A->vfgeo, smem};
+  vol_body<S>(valid, i, A->rhsQ_in + base, A->vgeo + kk * NQ * NG, s_nref, A->ws, accV, gm1,
+              inv2gm1, acc, wc);
 
   // ---------------- surface flux + LF penalty, in place over the face rows.
   // The face points of all TPW elements are spread over the whole wave (2
