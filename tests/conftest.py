@@ -14,6 +14,9 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def oracle():
+    """The C++ oracle; OpenMP over elements on up to 16 host threads (the GPU
+    box's CPU share), results independent of the thread count."""
     import pyoracle
     pyoracle.build()
+    pyoracle.set_threads(min(16, os.cpu_count() or 1))
     return pyoracle
