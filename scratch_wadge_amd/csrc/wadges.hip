@@ -707,12 +707,30 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
     for (int f = 0; f < NFLD; ++f) bufA[f * NQ + i] = rhs[f];
   }
   __syncthreads();
+  // Row i of VqPq serves all four products below: for the tetrahedral shapes with
+  // Nq <= 32 it is held in registers (the volume phase's registers are free by
+  // now; in 2D the register budget of the higher occupancy has no room), else it
+  // is read through L1.
+  constexpr bool ROW_IN_REGS = DIM == 3 && NQ <= 32;
+  double Prow[ROW_IN_REGS ? NQ : 1];
+  if constexpr (ROW_IN_REGS) {
+    if (valid)
+      for (int j = 0; j < NQ; ++j) Prow[j] = A->VqPq[i + j * NQ];
+  }
   auto matvec = [&](const double *in, double scale, double (&r)[NFLD]) {
     for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
+    if constexpr (ROW_IN_REGS) {
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const double P = scale * Prow[j];
+        for (int f = 0; f < NFLD; ++f) r[f] += P * in[f * NQ + j];
+      }
+    } else {
 #pragma unroll 8
-    for (int j = 0; j < NQ; ++j) {
-      const double P = scale * A->VqPq[i + j * NQ];
-      for (int f = 0; f < NFLD; ++f) r[f] += P * in[f * NQ + j];
+      for (int j = 0; j < NQ; ++j) {
+        const double P = scale * A->VqPq[i + j * NQ];
+        for (int f = 0; f < NFLD; ++f) r[f] += P * in[f * NQ + j];
+      }
     }
   };
   if (valid) {
