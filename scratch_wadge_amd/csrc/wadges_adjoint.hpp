@@ -91,7 +91,6 @@ rhs_vjp_kernel(int K, const double *__restrict__ vgeo, const double *__restrict_
   constexpr int NG = S::NG, TVT = S::TVT, VS = S::VS, VREC = S::VREC, FREC = S::FREC;
   constexpr int NPB = VjpSmem<S>::NPB, PER = VjpSmem<S>::per_elem, NFG = DIM + 1;
   constexpr int T = vf_steps<S>();
-  constexpr int MAXF = (NFN + NQ - 1) / NQ;  // face points per lane
   extern __shared__ double smem[];
   __shared__ double s_nref[S::NFACES][DIM];
 
@@ -169,22 +168,30 @@ rhs_vjp_kernel(int K, const double *__restrict__ vgeo, const double *__restrict_
   }
   __syncthreads();
 
-  // ---------------- phase 2: face records, s_bar, volume-face pairs
-  if (valid) {
-    for (int fn = i; fn < NFN; fn += NQ) {
-      double U[NFLD];
-      for (int f = 0; f < NFLD; ++f) U[f] = Uf[kk * NFN * NFLD + f * NFN + fn];
-      store_node<DIM>(frec, NFN, fn, make_node<DIM>(U, gm1));
-      const int face = fn / NFQ;
-      for (int d = 0; d < DIM; ++d) {
-        double b = 0.0;
-        for (int r = 0; r < DIM; ++r)
-          b = b + 0.5 * vfgeo[kk * NFN * NG + (r * DIM + d) * NFN + fn] * s_nref[face][r];
-        frec[(VS + d) * NFN + fn] = b;
-      }
-      for (int f = 0; f < NFLD; ++f) sbF[f * NFN + fn] = sbar[kk * NFN * NFLD + f * NFN + fn];
-      for (int c = 0; c < NPB; ++c) gF[c * NFN + fn] = 0.0;
+  // ---------------- phase 2: face records, s_bar, volume-face pairs.  Face
+  // point work of all teams is spread over the whole wave.
+  auto team_base = [&](int t) { return smem + t * PER; };
+  constexpr int OFF_FREC = NPB * NQ, OFF_SBF = OFF_FREC + FREC * NFN, OFF_GF = OFF_SBF + NFLD * NFN;
+  for (int p = lane; p < TPW * NFN; p += WAVE) {
+    const int t = p / NFN;
+    const int fn = p - t * NFN;
+    const size_t kt = (size_t)blockIdx.x * TPW + t;
+    if (kt >= (size_t)K) continue;
+    double *frt = team_base(t) + OFF_FREC;
+    double *sbt = team_base(t) + OFF_SBF;
+    double *gft = team_base(t) + OFF_GF;
+    double U[NFLD];
+    for (int f = 0; f < NFLD; ++f) U[f] = Uf[kt * NFN * NFLD + f * NFN + fn];
+    store_node<DIM>(frt, NFN, fn, make_node<DIM>(U, gm1));
+    const int face = fn / NFQ;
+    for (int d = 0; d < DIM; ++d) {
+      double b = 0.0;
+      for (int r = 0; r < DIM; ++r)
+        b = b + 0.5 * vfgeo[kt * NFN * NG + (r * DIM + d) * NFN + fn] * s_nref[face][r];
+      frt[(VS + d) * NFN + fn] = b;
     }
+    for (int f = 0; f < NFLD; ++f) sbt[f * NFN + fn] = sbar[kt * NFN * NFLD + f * NFN + fn];
+    for (int c = 0; c < NPB; ++c) gft[c * NFN + fn] = 0.0;
   }
   __syncthreads();
   if (valid) {
@@ -235,70 +242,73 @@ rhs_vjp_kernel(int K, const double *__restrict__ vgeo, const double *__restrict_
   }
   __syncthreads();
 
-  // ---------------- phase 3: surface terms at this lane's face points
-  double cons[MAXF][NFLD];
-  for (int m = 0; m < MAXF; ++m)
-    for (int f = 0; f < NFLD; ++f) cons[m][f] = 0.0;
-  if (valid) {
-    const double *g = fgeo + kk * NFN * NFG;
-    for (int m = 0; m < MAXF; ++m) {
-      const int fn = i + m * NQ;
-      if (fn >= NFN) continue;
-      const size_t idM = (size_t)fn + kk * NFN * NFLD;
-      const size_t idP = mapPq[kk * NFN + fn];
-      double UM[NFLD], UP[NFLD], sbM[NFLD];
-      for (int f = 0; f < NFLD; ++f) {
-        UM[f] = Uf[idM + (size_t)f * NFN];
-        UP[f] = Uf[idP + (size_t)f * NFN];
-        sbM[f] = sbF[f * NFN + fn];
-      }
-      const Node<DIM> a = make_node<DIM>(UM, gm1);
-      const Node<DIM> b = make_node<DIM>(UP, gm1);
-      double hnM[DIM];
-      for (int d = 0; d < DIM; ++d) hnM[d] = 0.5 * g[fn + d * NFN];
-      const double sJM = g[fn + DIM * NFN];
-      PrimBar<DIM> ga, gb;
-      primbar_zero<DIM>(ga);
-      primbar_zero<DIM>(gb);
-      flux_dot_vjp<DIM>(a, b, hnM, sbM, inv2gm1, ga, gb);
-      double dlM[NFLD], dlP[NFLD];
-      const double lamM = wavespeed<DIM>(UM, gamma, gm1, dlM);
-      const double lamP = wavespeed<DIM>(UP, gamma, gm1, dlP);
-      if (idP == idM) {
-        // no neighbour: u~P = u~M, zero jump; both flux arguments are this point
-        ga.rho += gb.rho;
-        for (int d = 0; d < DIM; ++d) ga.u[d] += gb.u[d];
-        ga.beta += gb.beta;
-      } else {
-        // own LF term  -ltau max(lamM, lamP) sJM (UP - UM) . sbM
-        const double LfM = ltau * fmax(lamM, lamP) * sJM;
-        double S1 = 0.0;
-        for (int f = 0; f < NFLD; ++f) S1 = S1 + sbM[f] * (UP[f] - UM[f]);
-        const double dM = (lamM >= lamP) ? -S1 * ltau * sJM : 0.0;
-        for (int f = 0; f < NFLD; ++f) cons[m][f] += LfM * sbM[f] + dM * dlM[f];
-        // the neighbour's terms at its matching point, where this point is the
-        // second flux argument and the P side of its penalty
-        const size_t eN = idP / ((size_t)NFN * NFLD);
-        const int pN = (int)(idP - eN * (size_t)NFN * NFLD);
-        const double *gN = fgeo + eN * NFN * NFG;
-        double hnP[DIM], sbP[NFLD];
-        for (int d = 0; d < DIM; ++d) hnP[d] = 0.5 * gN[pN + d * NFN];
-        const double sJP = gN[pN + DIM * NFN];
-        for (int f = 0; f < NFLD; ++f) sbP[f] = sbar[idP + (size_t)f * NFN];
-        PrimBar<DIM> gx;
-        primbar_zero<DIM>(gx);
-        flux_dot_vjp<DIM>(b, a, hnP, sbP, inv2gm1, gx, ga);
-        const double LfP = ltau * fmax(lamP, lamM) * sJP;
-        double S2 = 0.0;
-        for (int f = 0; f < NFLD; ++f) S2 = S2 + sbP[f] * (UM[f] - UP[f]);
-        const double dP = (lamM > lamP) ? -S2 * ltau * sJP : 0.0;
-        for (int f = 0; f < NFLD; ++f) cons[m][f] += -LfP * sbP[f] + dP * dlM[f];
-      }
-      // plain add: this lane owns face point fn after the barrier
-      gF[0 * NFN + fn] += ga.rho;
-      for (int d = 0; d < DIM; ++d) gF[(1 + d) * NFN + fn] += ga.u[d];
-      gF[(DIM + 1) * NFN + fn] += ga.beta;
+  // ---------------- phase 3: surface terms at the face points, spread over the
+  // whole wave.  A point's conservative-space part overwrites its s_bar slot
+  // (only this lane reads that slot from here on).
+  for (int p = lane; p < TPW * NFN; p += WAVE) {
+    const int t = p / NFN;
+    const int fn = p - t * NFN;
+    const size_t kt = (size_t)blockIdx.x * TPW + t;
+    if (kt >= (size_t)K) continue;
+    double *sbt = team_base(t) + OFF_SBF;
+    double *gft = team_base(t) + OFF_GF;
+    const double *g = fgeo + kt * NFN * NFG;
+    const size_t idM = (size_t)fn + kt * NFN * NFLD;
+    const size_t idP = mapPq[kt * NFN + fn];
+    double UM[NFLD], UP[NFLD], sbM[NFLD], cons[NFLD];
+    for (int f = 0; f < NFLD; ++f) {
+      UM[f] = Uf[idM + (size_t)f * NFN];
+      UP[f] = Uf[idP + (size_t)f * NFN];
+      sbM[f] = sbt[f * NFN + fn];
+      cons[f] = 0.0;
     }
+    const Node<DIM> a = make_node<DIM>(UM, gm1);
+    const Node<DIM> b = make_node<DIM>(UP, gm1);
+    double hnM[DIM];
+    for (int d = 0; d < DIM; ++d) hnM[d] = 0.5 * g[fn + d * NFN];
+    const double sJM = g[fn + DIM * NFN];
+    PrimBar<DIM> ga, gb;
+    primbar_zero<DIM>(ga);
+    primbar_zero<DIM>(gb);
+    flux_dot_vjp<DIM>(a, b, hnM, sbM, inv2gm1, ga, gb);
+    double dlM[NFLD], dlP[NFLD];
+    const double lamM = wavespeed<DIM>(UM, gamma, gm1, dlM);
+    const double lamP = wavespeed<DIM>(UP, gamma, gm1, dlP);
+    if (idP == idM) {
+      // no neighbour: u~P = u~M, zero jump; both flux arguments are this point
+      ga.rho += gb.rho;
+      for (int d = 0; d < DIM; ++d) ga.u[d] += gb.u[d];
+      ga.beta += gb.beta;
+    } else {
+      // own LF term  -ltau max(lamM, lamP) sJM (UP - UM) . sbM
+      const double LfM = ltau * fmax(lamM, lamP) * sJM;
+      double S1 = 0.0;
+      for (int f = 0; f < NFLD; ++f) S1 = S1 + sbM[f] * (UP[f] - UM[f]);
+      const double dM = (lamM >= lamP) ? -S1 * ltau * sJM : 0.0;
+      for (int f = 0; f < NFLD; ++f) cons[f] += LfM * sbM[f] + dM * dlM[f];
+      // the neighbour's terms at its matching point, where this point is the
+      // second flux argument and the P side of its penalty
+      const size_t eN = idP / ((size_t)NFN * NFLD);
+      const int pN = (int)(idP - eN * (size_t)NFN * NFLD);
+      const double *gN = fgeo + eN * NFN * NFG;
+      double hnP[DIM], sbP[NFLD];
+      for (int d = 0; d < DIM; ++d) hnP[d] = 0.5 * gN[pN + d * NFN];
+      const double sJP = gN[pN + DIM * NFN];
+      for (int f = 0; f < NFLD; ++f) sbP[f] = sbar[idP + (size_t)f * NFN];
+      PrimBar<DIM> gx;
+      primbar_zero<DIM>(gx);
+      flux_dot_vjp<DIM>(b, a, hnP, sbP, inv2gm1, gx, ga);
+      const double LfP = ltau * fmax(lamP, lamM) * sJP;
+      double S2 = 0.0;
+      for (int f = 0; f < NFLD; ++f) S2 = S2 + sbP[f] * (UM[f] - UP[f]);
+      const double dP = (lamM > lamP) ? -S2 * ltau * sJP : 0.0;
+      for (int f = 0; f < NFLD; ++f) cons[f] += -LfP * sbP[f] + dP * dlM[f];
+    }
+    // plain adds: this lane owns face point (t, fn) after the barrier
+    gft[0 * NFN + fn] += ga.rho;
+    for (int d = 0; d < DIM; ++d) gft[(1 + d) * NFN + fn] += ga.u[d];
+    gft[(DIM + 1) * NFN + fn] += ga.beta;
+    for (int f = 0; f < NFLD; ++f) sbt[f * NFN + fn] = cons[f];
   }
   __syncthreads();
 
@@ -308,16 +318,22 @@ rhs_vjp_kernel(int K, const double *__restrict__ vgeo, const double *__restrict_
     double Ub[NFLD];
     primbar_to_cons<DIM>(me, gI, gm1, Ub);
     for (int f = 0; f < NFLD; ++f) uqb[kk * NQ * NFLD + f * NQ + i] = Ub[f];
-    for (int m = 0; m < MAXF; ++m) {
-      const int fn = i + m * NQ;
-      if (fn >= NFN) continue;
-      const Node<DIM> nf = load_node<DIM>(frec, NFN, fn);
-      PrimBar<DIM> gf;
-      primbar_zero<DIM>(gf);
-      add_lds_primbar<DIM>(gf, gF, NFN, fn);
-      primbar_to_cons<DIM>(nf, gf, gm1, Ub);
-      for (int f = 0; f < NFLD; ++f) ufb[kk * NFN * NFLD + f * NFN + fn] = Ub[f] + cons[m][f];
-    }
+  }
+  for (int p = lane; p < TPW * NFN; p += WAVE) {
+    const int t = p / NFN;
+    const int fn = p - t * NFN;
+    const size_t kt = (size_t)blockIdx.x * TPW + t;
+    if (kt >= (size_t)K) continue;
+    const double *frt = team_base(t) + OFF_FREC;
+    const double *sbt = team_base(t) + OFF_SBF;
+    double *gft = team_base(t) + OFF_GF;
+    const Node<DIM> nf = load_node<DIM>(frt, NFN, fn);
+    PrimBar<DIM> gf;
+    primbar_zero<DIM>(gf);
+    add_lds_primbar<DIM>(gf, gft, NFN, fn);
+    double Ub[NFLD];
+    primbar_to_cons<DIM>(nf, gf, gm1, Ub);
+    for (int f = 0; f < NFLD; ++f) ufb[kt * NFN * NFLD + f * NFN + fn] = Ub[f] + sbt[f * NFN + fn];
   }
 }
 
