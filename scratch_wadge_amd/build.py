@@ -21,6 +21,11 @@ HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # no packed FP32 instructions in any generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds are fused within a source expression only (a * b + c), never
+    # across statements by the backend: the fusion then does not depend on the
+    # code a routine is inlined into, so every kernel that runs the fused stage
+    # body (single-problem, ensemble flat and persistent) computes the same bits
+    "-ffp-contract=on",
 ]
 
 

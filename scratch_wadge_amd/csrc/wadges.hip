@@ -49,6 +49,15 @@ constexpr int WAVE = 64;
 
 template <int A, int B> struct Max { static constexpr int value = A > B ? A : B; };
 
+// Volume rows per lane of the forward volume block.  With R > 1 a lane holds
+// rows l, l + LPE, ..., so a partner record read from LDS, and the partner's
+// LDS accumulation, serve R rows, and more elements share a wave: 3D N = 3
+// (Nq = 23) packs 5 elements of 12 lanes (60 lanes, 23 of 24 rows used)
+// instead of 2 elements of 23 lanes (46 lanes).  R > 1 requires the lanes of an
+// element to fit on one face (LPE <= Nfq): the volume-face block then walks
+// the faces one at a time.
+constexpr int vol_rows(int dim, int nq, int nfq) { return (dim == 3 && nq == 23 && nfq == 12) ? 2 : 1; }
+
 // Compile-time shape of one (dim, Nq, Nfq) instance.
 template <int DIM_, int NQ_, int NFQ_> struct Shape {
   static constexpr int DIM = DIM_;
@@ -67,7 +76,13 @@ template <int DIM_, int NQ_, int NFQ_> struct Shape {
   // workspace layout (doubles)
   static constexpr int WS_D = 2 * DIM * TVT * NQ;   // D fwd / bwd
   static constexpr int WS_W = 2 * L * NQ;           // +1/2 VqLq, -1/2 VfPq
-  static constexpr int WS = WS_D + WS_W;
+  // forward volume block: rows per lane, lanes per element, elements per wave
+  static constexpr int R = vol_rows(DIM_, NQ_, NFQ_);
+  static constexpr int LPE = (NQ + R - 1) / R;
+  static constexpr int TPW = WAVE / LPE;
+  static constexpr int WS_R = R > 1 ? 2 * NFN * NQ : 0;   // face-walk weights (R > 1)
+  static constexpr int WS = WS_D + WS_W + WS_R;
+  static_assert(R == 1 || (NQ <= NFN && LPE <= NFQ), "R > 1 needs one element's lanes on one face");
 };
 
 // ------------------------------------------------------------------ pair schedules
@@ -102,12 +117,15 @@ template <class S> __device__ __forceinline__ int vf_partner(int t, int i) {
 
 template <class S> constexpr int vf_steps() { return face_chunked<S>() ? S::NFN : S::L; }
 
-// Face-chunked tetrahedra whose lanes form at most two groups visit the faces in
-// two halves (faces 0-1, then 2-3), the two groups on different faces of the
-// half at every step; only the current half's face records are held in LDS,
-// which raises occupancy (3D N = 3: 4 instead of 2.75 waves per SIMD).  The
-// steps t and their partners are those of vf_partner, taken in another order.
-template <class S> constexpr int face_halves() {
+// The volume-face block takes the face records in parts, holding only the
+// current part's records in LDS.  With one row per lane, face-chunked
+// tetrahedra whose lanes form at most two groups visit the faces in two halves
+// (faces 0-1, then 2-3), the two groups on different faces of the half at every
+// step (the steps t and their partners are those of vf_partner, taken in another
+// order).  With R > 1 rows per lane an element's lanes form one group, which
+// walks the faces one at a time.
+template <class S> constexpr int face_parts() {
+  if constexpr (S::R > 1) return S::NFACES;
   return (face_chunked<S>() && S::NFACES == 4 && (S::NQ + S::NFQ - 1) / S::NFQ <= 2) ? 2 : 1;
 }
 
@@ -137,6 +155,17 @@ __global__ void prep_vol_tables(const double *__restrict__ Drq, const double *__
     double v = 0.0;
     if (f < NFN) v = s == 0 ? 0.5 * VqLq[i + f * NQ] : -0.5 * VfPq[f + i * NFN];
     ws[idx] = v;
+  } else if (S::R > 1 && idx >= S::WS_D + S::WS_W && idx < S::WS) {
+    // face walk of R > 1: step t = face * NFQ + m, row i on lane l = i mod LPE
+    // visits point (l + m) mod NFQ of that face
+    const int k = idx - S::WS_D - S::WS_W;
+    const int i = k % NQ;
+    const int t = (k / NQ) % NFN;
+    const int s = k / (NQ * NFN);
+    int q = i % S::LPE + t % S::NFQ;
+    if (q >= S::NFQ) q -= S::NFQ;
+    const int f = (t / S::NFQ) * S::NFQ + q;
+    ws[idx] = s == 0 ? 0.5 * VqLq[i + f * NQ] : -0.5 * VfPq[f + i * NFN];
   }
 }
 
@@ -146,13 +175,14 @@ __global__ void prep_vol_tables(const double *__restrict__ Drq, const double *__
 // volume-node records (volume-volume block) and then for the face-node records
 // and face accumulator (volume-face block).
 template <class S> struct VolSmem {
-  static constexpr int H = face_halves<S>();
+  static constexpr int H = face_parts<S>();
   static constexpr int NH = S::NFN / H;                    // face records held at a time
   static constexpr int acc_v = S::NFLD * S::NQ;
   static constexpr int phase1 = S::VREC * S::NQ;
   static constexpr int phase2 = S::FREC * NH + S::NFLD * S::NFN;
   static constexpr int per_elem = acc_v + (phase1 > phase2 ? phase1 : phase2);
   static constexpr int accF_off = acc_v + S::FREC * NH;    // face accumulators [NFLD][NFN]
+  static_assert(per_elem >= 2 * acc_v, "the fused stage's update needs two [NFLD][NQ] buffers");
 };
 
 // Waves per SIMD that the LDS of a one-wave workgroup of `bytes` allows (160 KiB
@@ -167,7 +197,7 @@ constexpr int lds_waves_per_simd(int bytes) {
 }
 
 template <class S> constexpr int vol_waves() {
-  return lds_waves_per_simd((WAVE / S::NQ) * VolSmem<S>::per_elem * 8 + S::NFACES * S::DIM * 8);
+  return lds_waves_per_simd(S::TPW * VolSmem<S>::per_elem * 8 + S::NFACES * S::DIM * 8);
 }
 // Lower bound handed to the compiler: at most four waves (128 VGPRs), so that
 // shapes whose LDS would allow more are not squeezed into spills.
@@ -204,69 +234,81 @@ struct WaveCtx {
   double *smem;        // the workgroup's dynamic LDS (team t at t * per_elem)
 };
 
-// The volume term of one element (both coupling blocks).  On return acc holds
-// the register part of volume row i, accV the LDS part (partner contributions),
-// accF the face rows (rhsQf).  Uq is the element's projected volume state, vg
-// its volume geofacs; the face state and geofacs of the wave's elements come
-// through wc.  Every lane of the workgroup must call this (barriers).
+// The volume term of one element (both coupling blocks).  Lane l of the
+// element's team owns rows l + r * LPE (r < R, those below NQ).  On return
+// acc[r] holds the register part of its row r, accV the LDS part (partner
+// contributions), accF the face rows (rhsQf).  Uq is the element's projected
+// volume state, vg its volume geofacs; the face state and geofacs of the wave's
+// elements come through wc.  Every lane of the workgroup must call this
+// (barriers).
 template <class S>
-__device__ __forceinline__ void vol_body(bool valid, int i, const double *Uq, const double *vg,
+__device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, const double *vg,
                                          const double (*s_nref)[S::DIM], const double *ws,
                                          double *accV, double gm1, double inv2gm1,
-                                         double (&acc)[S::NFLD], const WaveCtx &wc) {
+                                         double (&acc)[S::R][S::NFLD], const WaveCtx &wc) {
   constexpr int DIM = S::DIM, NQ = S::NQ, NFQ = S::NFQ, NFN = S::NFN, NFLD = S::NFLD;
-  constexpr int NG = S::NG, TVT = S::TVT, VS = S::VS, FREC = S::FREC;
+  constexpr int NG = S::NG, TVT = S::TVT, VS = S::VS, R = S::R, LPE = S::LPE;
   constexpr int T = vf_steps<S>();
   constexpr int H = VolSmem<S>::H, NH = VolSmem<S>::NH;
   double *vrec = accV + NFLD * NQ;         // phase 1
   double *frec = vrec;                      // phase 2: records of NH face points
   double *accF = accV + VolSmem<S>::accF_off;
 
+  bool rv[R];                               // row r of this lane exists
+  for (int r = 0; r < R; ++r) rv[r] = valid && (r == 0 || l + r * LPE < NQ);
+
   // ---------------- phase 1: volume records, volume-volume block
-  Node<DIM> me;
-  double hG[NG];
-  if (valid) {
+  Node<DIM> me[R];
+  double hG[R][NG];
+  for (int r = 0; r < R; ++r) {
+    if (!rv[r]) continue;
+    const int i = l + r * LPE;
     double U[NFLD];
     for (int f = 0; f < NFLD; ++f) U[f] = Uq[i + f * NQ];
-    me = make_node<DIM>(U, gm1);
-    for (int c = 0; c < NG; ++c) hG[c] = 0.5 * vg[i + c * NQ];
-    store_node<DIM>(vrec, NQ, i, me);
-    for (int c = 0; c < NG; ++c) vrec[(VS + c) * NQ + i] = hG[c];
+    me[r] = make_node<DIM>(U, gm1);
+    for (int c = 0; c < NG; ++c) hG[r][c] = 0.5 * vg[i + c * NQ];
+    store_node<DIM>(vrec, NQ, i, me[r]);
+    for (int c = 0; c < NG; ++c) vrec[(VS + c) * NQ + i] = hG[r][c];
     for (int f = 0; f < NFLD; ++f) accV[f * NQ + i] = 0.0;
   }
   __syncthreads();
 
-  for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
+  for (int r = 0; r < R; ++r)
+    for (int f = 0; f < NFLD; ++f) acc[r][f] = 0.0;
 
   if (valid) {
     const double *Dfw = ws;
     const double *Dbw = ws + DIM * TVT * NQ;
     for (int t = 0; t < TVT; ++t) {
       if (2 * t > NQ) break;
-      if (2 * t == NQ && i >= NQ / 2) continue;  // even NQ: middle pair once
-      int j = i + t;
-      if (j >= NQ) j -= NQ;
-      const Node<DIM> pj = load_node<DIM>(vrec, NQ, j);
-      double Ga[NG];
-      for (int c = 0; c < NG; ++c) Ga[c] = hG[c] + vrec[(VS + c) * NQ + j];
-      const PairFlux<DIM> P = pair_flux<DIM>(me, pj, inv2gm1);
-      double Dx[DIM];
-      for (int d = 0; d < DIM; ++d) Dx[d] = 0.0;
-      for (int r = 0; r < DIM; ++r) {
-        const double Dr = Dfw[(r * TVT + t) * NQ + i];
-        for (int d = 0; d < DIM; ++d) Dx[d] = Dx[d] + Dr * Ga[r * DIM + d];
-      }
-      double c[NFLD];
-      flux_dot<DIM>(P, Dx, c);
-      for (int f = 0; f < NFLD; ++f) acc[f] += c[f];
-      if (t > 0) {
+      for (int r = 0; r < R; ++r) {
+        if (!rv[r]) continue;
+        const int i = l + r * LPE;
+        if (2 * t == NQ && i >= NQ / 2) continue;  // even NQ: middle pair once
+        int j = i + t;
+        if (j >= NQ) j -= NQ;
+        const Node<DIM> pj = load_node<DIM>(vrec, NQ, j);
+        double Ga[NG];
+        for (int c = 0; c < NG; ++c) Ga[c] = hG[r][c] + vrec[(VS + c) * NQ + j];
+        const PairFlux<DIM> P = pair_flux<DIM>(me[r], pj, inv2gm1);
+        double Dx[DIM];
         for (int d = 0; d < DIM; ++d) Dx[d] = 0.0;
-        for (int r = 0; r < DIM; ++r) {
-          const double Dr = Dbw[(r * TVT + t) * NQ + i];
-          for (int d = 0; d < DIM; ++d) Dx[d] = Dx[d] + Dr * Ga[r * DIM + d];
+        for (int q = 0; q < DIM; ++q) {
+          const double Dr = Dfw[(q * TVT + t) * NQ + i];
+          for (int d = 0; d < DIM; ++d) Dx[d] = Dx[d] + Dr * Ga[q * DIM + d];
         }
+        double c[NFLD];
         flux_dot<DIM>(P, Dx, c);
-        for (int f = 0; f < NFLD; ++f) lds_add(&accV[f * NQ + j], c[f]);
+        for (int f = 0; f < NFLD; ++f) acc[r][f] += c[f];
+        if (t > 0) {
+          for (int d = 0; d < DIM; ++d) Dx[d] = 0.0;
+          for (int q = 0; q < DIM; ++q) {
+            const double Dr = Dbw[(q * TVT + t) * NQ + i];
+            for (int d = 0; d < DIM; ++d) Dx[d] = Dx[d] + Dr * Ga[q * DIM + d];
+          }
+          flux_dot<DIM>(P, Dx, c);
+          for (int f = 0; f < NFLD; ++f) lds_add(&accV[f * NQ + j], c[f]);
+        }
       }
     }
   }
@@ -275,33 +317,16 @@ __device__ __forceinline__ void vol_body(bool valid, int i, const double *Uq, co
   // ---------------- phase 2: face records (overwrite the volume records),
   // volume-face block, in H parts of NH face points.  Records and accumulator
   // clearing are spread over the whole wave (all teams' face points together).
-  constexpr int TPW = WAVE / NQ, PER = VolSmem<S>::per_elem;
+  constexpr int TPW = S::TPW, PER = VolSmem<S>::per_elem;
   for (int p = wc.lane; p < TPW * NFN * NFLD; p += WAVE) {
     const int t = p / (NFN * NFLD);
     wc.smem[t * PER + VolSmem<S>::accF_off + (p - t * NFN * NFLD)] = 0.0;
   }
-  const double *W1 = ws + S::WS_D;
-  const double *W2 = W1 + T * NQ;
-  // one volume-face pair: step t, partner face point fn (record fl), own-side normal part a
-  auto vf_pair = [&](int t, int fn, int fl, const double (&a)[DIM]) {
-    const Node<DIM> pf = load_node<DIM>(frec, NH, fl);
-    double n[DIM];
-    for (int d = 0; d < DIM; ++d) n[d] = a[d] + frec[(VS + d) * NH + fl];
-    const PairFlux<DIM> P = pair_flux<DIM>(me, pf, inv2gm1);
-    double Fn[NFLD];
-    flux_dot<DIM>(P, n, Fn);
-    const double w1 = W1[t * NQ + i];
-    const double w2 = W2[t * NQ + i];
-    for (int f = 0; f < NFLD; ++f) {
-      acc[f] += w1 * Fn[f];
-      lds_add(&accF[f * NFN + fn], w2 * Fn[f]);
-    }
-  };
-  // own-side part of the averaged normal on a face: sum_r (G_rd / 2) nref_r
-  auto own_normal = [&](int face, double (&a)[DIM]) {
+  // own-side part of the averaged normal of row r on a face: sum_q (G_qd / 2) nref_q
+  auto own_normal = [&](int r, int face, double (&a)[DIM]) {
     for (int d = 0; d < DIM; ++d) {
       double v = 0.0;
-      for (int r = 0; r < DIM; ++r) v = v + hG[r * DIM + d] * s_nref[face][r];
+      for (int q = 0; q < DIM; ++q) v = v + hG[r][q * DIM + d] * s_nref[face][q];
       a[d] = v;
     }
   };
@@ -321,45 +346,98 @@ __device__ __forceinline__ void vol_body(bool valid, int i, const double *Uq, co
       const int face = fn / NFQ;
       for (int d = 0; d < DIM; ++d) {
         double b = 0.0;
-        for (int r = 0; r < DIM; ++r) b = b + 0.5 * vfgt[fn + (r * DIM + d) * NFN] * s_nref[face][r];
+        for (int q = 0; q < DIM; ++q) b = b + 0.5 * vfgt[fn + (q * DIM + d) * NFN] * s_nref[face][q];
         frect[(VS + d) * NH + fl] = b;
       }
     }
     __syncthreads();
     if (valid) {
-      if constexpr (face_chunked<S>()) {
-        const int g = i / NFQ, o = i - (i / NFQ) * NFQ;
-        constexpr int FH = S::NFACES / H;            // faces per part
-        for (int u = 0; u < FH; ++u) {
-          // face of this group at part-step u; the chunk c of vf_partner that visits it
-          int fh = u + g;
-          while (fh >= FH) fh -= FH;
-          const int face = h * FH + fh;
-          int c = face - g;
-          if (c < 0) c += S::NFACES;
-          double a[DIM];
-          own_normal(face, a);
-          for (int m = 0; m < NFQ; ++m) {
-            int node = o + m;
-            if (node >= NFQ) node -= NFQ;
-            const int fn = face * NFQ + node;
-            vf_pair(c * NFQ + m, fn, fn - h * NH, a);
+      if constexpr (R > 1) {
+        // part h is face h; at step m lane l visits its point (l + m) mod NFQ,
+        // and the one record read serves all rows of the lane
+        const double *W1 = ws + S::WS_D + S::WS_W;
+        const double *W2 = W1 + NFN * NQ;
+        double a[R][DIM];
+        for (int r = 0; r < R; ++r)
+          if (rv[r]) own_normal(r, h, a[r]);
+        for (int m = 0; m < NFQ; ++m) {
+          int q = l + m;
+          if (q >= NFQ) q -= NFQ;
+          const Node<DIM> pf = load_node<DIM>(frec, NH, q);
+          double b[DIM];
+          for (int d = 0; d < DIM; ++d) b[d] = frec[(VS + d) * NH + q];
+          double sF[NFLD];
+          for (int f = 0; f < NFLD; ++f) sF[f] = 0.0;
+          const int t = h * NFQ + m;
+          for (int r = 0; r < R; ++r) {
+            if (!rv[r]) continue;
+            const int i = l + r * LPE;
+            double n[DIM];
+            for (int d = 0; d < DIM; ++d) n[d] = a[r][d] + b[d];
+            const PairFlux<DIM> P = pair_flux<DIM>(me[r], pf, inv2gm1);
+            double Fn[NFLD];
+            flux_dot<DIM>(P, n, Fn);
+            const double w1 = W1[t * NQ + i];
+            const double w2 = W2[t * NQ + i];
+            for (int f = 0; f < NFLD; ++f) {
+              acc[r][f] += w1 * Fn[f];
+              sF[f] += w2 * Fn[f];
+            }
           }
+          for (int f = 0; f < NFLD; ++f) lds_add(&accF[f * NFN + h * NFQ + q], sF[f]);
         }
       } else {
-        for (int t = 0; t < T; ++t) {
-          const int fn = vf_partner<S>(t, i);
-          if (fn >= NFN) continue;
-          double a[DIM];
-          own_normal(fn / NFQ, a);
-          vf_pair(t, fn, fn, a);
+        const int i = l;
+        const double *W1 = ws + S::WS_D;
+        const double *W2 = W1 + T * NQ;
+        // one volume-face pair: step t, partner face point fn (record fl), own-side normal part a
+        auto vf_pair = [&](int t, int fn, int fl, const double (&a)[DIM]) {
+          const Node<DIM> pf = load_node<DIM>(frec, NH, fl);
+          double n[DIM];
+          for (int d = 0; d < DIM; ++d) n[d] = a[d] + frec[(VS + d) * NH + fl];
+          const PairFlux<DIM> P = pair_flux<DIM>(me[0], pf, inv2gm1);
+          double Fn[NFLD];
+          flux_dot<DIM>(P, n, Fn);
+          const double w1 = W1[t * NQ + i];
+          const double w2 = W2[t * NQ + i];
+          for (int f = 0; f < NFLD; ++f) {
+            acc[0][f] += w1 * Fn[f];
+            lds_add(&accF[f * NFN + fn], w2 * Fn[f]);
+          }
+        };
+        if constexpr (face_chunked<S>()) {
+          const int g = i / NFQ, o = i - (i / NFQ) * NFQ;
+          constexpr int FH = S::NFACES / H;            // faces per part
+          for (int u = 0; u < FH; ++u) {
+            // face of this group at part-step u; the chunk c of vf_partner that visits it
+            int fh = u + g;
+            while (fh >= FH) fh -= FH;
+            const int face = h * FH + fh;
+            int c = face - g;
+            if (c < 0) c += S::NFACES;
+            double a[DIM];
+            own_normal(0, face, a);
+            for (int m = 0; m < NFQ; ++m) {
+              int node = o + m;
+              if (node >= NFQ) node -= NFQ;
+              const int fn = face * NFQ + node;
+              vf_pair(c * NFQ + m, fn, fn - h * NH, a);
+            }
+          }
+        } else {
+          for (int t = 0; t < T; ++t) {
+            const int fn = vf_partner<S>(t, i);
+            if (fn >= NFN) continue;
+            double a[DIM];
+            own_normal(0, fn / NFQ, a);
+            vf_pair(t, fn, fn, a);
+          }
         }
       }
     }
     if (H > 1) __syncthreads();   // records are overwritten by the next part
   }
   __syncthreads();
-
 }
 
 template <class S>
@@ -372,21 +450,22 @@ __device__ __forceinline__ void load_nref(int lane, const double *nrJ, const dou
   }
 }
 
-template <class S, int TPW>
+template <class S>
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(vol_waves_min<S>(), vol_waves<S>())))
 vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vfgeo,
            const double *__restrict__ nrJ, const double *__restrict__ nsJ,
            const double *__restrict__ ntJ, const double *__restrict__ ws,
            const double *__restrict__ Qf, double *rhsQ, double *__restrict__ rhsQf,
            double gm1, double inv2gm1) {
-  constexpr int NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD, NG = S::NG, FREC = S::FREC;
+  constexpr int NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD, NG = S::NG;
+  constexpr int R = S::R, LPE = S::LPE, TPW = S::TPW;
   constexpr int PER = VolSmem<S>::per_elem;
   extern __shared__ double smem[];
   __shared__ double s_nref[S::NFACES][S::DIM];
 
   const int lane = threadIdx.x;
-  const int team = lane / NQ;
-  const int i = lane - team * NQ;
+  const int team = lane / LPE;
+  const int l = lane - team * LPE;
   const bool active = team < TPW;
   const int k = blockIdx.x * TPW + team;
   const bool valid = active && k < K;
@@ -394,17 +473,21 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
 
   load_nref<S>(lane, nrJ, nsJ, ntJ, s_nref);
   double *accV = smem + (active ? team : 0) * PER;
-  double acc[NFLD];
+  double acc[R][NFLD];
   const WaveCtx wc{lane, (int)blockIdx.x * TPW, K, Qf, vfgeo, smem};
-  vol_body<S>(valid, i, rhsQ + kk * NQ * NFLD, vgeo + kk * NQ * NG, s_nref, ws, accV, gm1,
+  vol_body<S>(valid, l, rhsQ + kk * NQ * NFLD, vgeo + kk * NQ * NG, s_nref, ws, accV, gm1,
               inv2gm1, acc, wc);
   const double *accF = accV + VolSmem<S>::accF_off;
 
   if (valid) {
     double *dst = rhsQ + (size_t)k * NQ * NFLD;
-    for (int f = 0; f < NFLD; ++f) dst[i + f * NQ] = acc[f] + accV[f * NQ + i];
+    for (int r = 0; r < R; ++r) {
+      const int i = l + r * LPE;
+      if (r > 0 && i >= NQ) continue;
+      for (int f = 0; f < NFLD; ++f) dst[i + f * NQ] = acc[r][f] + accV[f * NQ + i];
+    }
     double *dstf = rhsQf + (size_t)k * NFN * NFLD;
-    for (int fn = i; fn < NFN; fn += NQ)
+    for (int fn = l; fn < NFN; fn += LPE)
       for (int f = 0; f < NFLD; ++f) dstf[fn + f * NFN] = accF[f * NFN + fn];
   }
 }
@@ -616,41 +699,43 @@ struct StageArgs {
 // pointers loaded from there are known to point to global memory.
 typedef __attribute__((address_space(4))) const StageArgs ConstArgs;
 
-// One fused stage for the TPW elements kfirst .. kfirst+TPW-1 (one team of NQ
-// lanes each) of the mesh described by A.  All lanes of the workgroup call it.
-// fa, fb and the face-trace buffers come separately from A so that an ensemble
-// can keep A in device memory and vary them per launch.  lane is threadIdx.x
-// (passed in so that a caller looping over tiles can hand an opaque copy).
-template <class S, int TPW>
+// One fused stage for the S::TPW elements kfirst .. kfirst+TPW-1 (one team of
+// S::LPE lanes each, lane l holding rows l + r * LPE) of the mesh described by
+// A.  All lanes of the workgroup call it.  fa, fb and the face-trace buffers
+// come separately from A so that an ensemble can keep A in device memory and
+// vary them per launch.  lane is threadIdx.x (passed in so that a caller
+// looping over tiles can hand an opaque copy).
+template <class S>
 __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double fb,
                                            const double *__restrict__ Qf_in,
                                            double *__restrict__ Qf_out, int kfirst, int lane,
                                            double (*s_nref)[S::DIM], double *smem) {
   constexpr int DIM = S::DIM, NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD, NG = S::NG;
-  constexpr int FREC = S::FREC, NFG = DIM + 1;
+  constexpr int NFG = DIM + 1, R = S::R, LPE = S::LPE, TPW = S::TPW;
   constexpr int PER = VolSmem<S>::per_elem;
   const double gamma = A->gamma, gm1 = A->gm1, inv2gm1 = A->inv2gm1, ltau = A->ltau, lgm1 = A->lgm1;
 
-  const int team = lane / NQ;
-  const int i = lane - team * NQ;
+  const int team = lane / LPE;
+  const int l = lane - team * LPE;
   const bool active = team < TPW;
   const int k = kfirst + team;
   const bool valid = active && k < A->K;
   const size_t kk = valid ? (size_t)k : 0;
   const size_t base = kk * NQ * NFLD;
   const double fdt = A->fdt;
+  bool rv[R];                     // row r = l + r * LPE of this lane exists
+  for (int r = 0; r < R; ++r) rv[r] = valid && (r == 0 || l + r * LPE < NQ);
 
   load_nref<S>(lane, A->nrJ, A->nsJ, A->ntJ, s_nref);
   double *accV = smem + (active ? team : 0) * PER;
   double *accF = accV + VolSmem<S>::accF_off;
-  double acc[NFLD];
+  double acc[R][NFLD];
   const WaveCtx wc{lane, kfirst, A->K, Qf_in, A->vfgeo, smem};
-  vol_body<S>(valid, i, A->rhsQ_in + base, A->vgeo + kk * NQ * NG, s_nref, A->ws, accV, gm1,
+  vol_body<S>(valid, l, A->rhsQ_in + base, A->vgeo + kk * NQ * NG, s_nref, A->ws, accV, gm1,
               inv2gm1, acc, wc);
 
   // ---------------- surface flux + LF penalty, in place over the face rows.
-  // The face points of all TPW elements are spread over the whole wave (2
-  // passes instead of 3 at Nq = 23, Nf = 48).
+  // The face points of all TPW elements are spread over the whole wave.
   for (int p = lane; p < TPW * NFN; p += WAVE) {
     const int t = p / NFN;
     const int fn = p - t * NFN;
@@ -683,86 +768,124 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
       accFt[f * NFN + fn] = 0.5 * Fn[f] + accFt[f * NFN + fn] - Lfc * (UP[f] - UM[f]);
   }
   __syncthreads();
-  // ---------------- RHS row i = volume part + lift of the face terms
-  double rhs[NFLD];
+  // ---------------- RHS rows = volume part + lift of the face terms
+  // (rows that do not exist read row 0's operator entries and are never stored)
+  int ir[R];
+  for (int r = 0; r < R; ++r) ir[r] = rv[r] ? l + r * LPE : 0;
+  double rhs[R][NFLD];
   if (valid) {
-    double val[NFLD];
-    for (int f = 0; f < NFLD; ++f) val[f] = 0.0;
+    double val[R][NFLD];
+    for (int r = 0; r < R; ++r)
+      for (int f = 0; f < NFLD; ++f) val[r][f] = 0.0;
 #pragma unroll 8
     for (int j = 0; j < NFN; ++j) {
-      const double L = A->VqLq[i + j * NQ];
-      for (int f = 0; f < NFLD; ++f) val[f] += L * accF[f * NFN + j];
+      double x[NFLD];
+      for (int f = 0; f < NFLD; ++f) x[f] = accF[f * NFN + j];
+      for (int r = 0; r < R; ++r) {
+        const double L = A->VqLq[ir[r] + j * NQ];
+        for (int f = 0; f < NFLD; ++f) val[r][f] += L * x[f];
+      }
     }
-    for (int f = 0; f < NFLD; ++f) rhs[f] = (acc[f] + accV[f * NQ + i]) + val[f];
+    for (int r = 0; r < R; ++r)
+      for (int f = 0; f < NFLD; ++f) rhs[r][f] = (acc[r][f] + accV[f * NQ + ir[r]]) + val[r][f];
   }
   __syncthreads();
 
   // ---------------- update (same sequence as update_kernel)
   double *bufA = accV;                // [NFLD][NQ]
   double *bufB = accV + NFLD * NQ;    // [NFLD][NQ]
-  double J = 1.0, invJ = 1.0;
-  if (valid) {
-    J = A->Jq[kk * NQ + i];
-    invJ = 1.0 / J;
-    for (int f = 0; f < NFLD; ++f) bufA[f * NQ + i] = rhs[f];
+  double J[R], invJ[R];
+  for (int r = 0; r < R; ++r) {
+    J[r] = 1.0;
+    invJ[r] = 1.0;
+    if (rv[r]) {
+      J[r] = A->Jq[kk * NQ + ir[r]];
+      invJ[r] = 1.0 / J[r];
+      for (int f = 0; f < NFLD; ++f) bufA[f * NQ + ir[r]] = rhs[r][f];
+    }
   }
   __syncthreads();
   // Row i of VqPq serves all four products below: for the tetrahedral shapes with
-  // Nq <= 32 it is held in registers (the volume phase's registers are free by
-  // now; in 2D the register budget of the higher occupancy has no room), else it
-  // is read through L1.
-  constexpr bool ROW_IN_REGS = DIM == 3 && NQ <= 32;
-  double Prow[ROW_IN_REGS ? NQ : 1];
+  // Nq <= 32 and one row per lane it is held in registers (the volume phase's
+  // registers are free by now; in 2D the register budget of the higher
+  // occupancy has no room), else it is read through L1.
+  constexpr bool ROW_IN_REGS = DIM == 3 && R * NQ <= 32;
+  double Prow[ROW_IN_REGS ? R : 1][ROW_IN_REGS ? NQ : 1];
   if constexpr (ROW_IN_REGS) {
     if (valid)
-      for (int j = 0; j < NQ; ++j) Prow[j] = A->VqPq[i + j * NQ];
+      for (int r = 0; r < R; ++r)
+        for (int j = 0; j < NQ; ++j) Prow[r][j] = A->VqPq[ir[r] + j * NQ];
   }
-  auto matvec = [&](const double *in, double scale, double (&r)[NFLD]) {
-    for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
+  auto matvec = [&](const double *in, double scale, double (&res)[R][NFLD]) {
+    for (int r = 0; r < R; ++r)
+      for (int f = 0; f < NFLD; ++f) res[r][f] = 0.0;
     if constexpr (ROW_IN_REGS) {
 #pragma unroll
       for (int j = 0; j < NQ; ++j) {
-        const double P = scale * Prow[j];
-        for (int f = 0; f < NFLD; ++f) r[f] += P * in[f * NQ + j];
+        double x[NFLD];
+        for (int f = 0; f < NFLD; ++f) x[f] = in[f * NQ + j];
+        for (int r = 0; r < R; ++r) {
+          const double P = scale * Prow[r][j];
+          for (int f = 0; f < NFLD; ++f) res[r][f] += P * x[f];
+        }
       }
     } else {
 #pragma unroll 8
       for (int j = 0; j < NQ; ++j) {
-        const double P = scale * A->VqPq[i + j * NQ];
-        for (int f = 0; f < NFLD; ++f) r[f] += P * in[f * NQ + j];
+        double x[NFLD];
+        for (int f = 0; f < NFLD; ++f) x[f] = in[f * NQ + j];
+        for (int r = 0; r < R; ++r) {
+          const double P = scale * A->VqPq[ir[r] + j * NQ];
+          for (int f = 0; f < NFLD; ++f) res[r][f] += P * x[f];
+        }
       }
     }
   };
   if (valid) {
-    double r[NFLD];
-    matvec(bufA, -2.0, r);
-    for (int f = 0; f < NFLD; ++f) bufB[f * NQ + i] = (DIM == 3) ? r[f] * invJ : r[f] / J;
-  }
-  __syncthreads();
-  if (valid) {
-    double r[NFLD], U[NFLD], V[NFLD];
-    matvec(bufB, 1.0, r);
-    for (int f = 0; f < NFLD; ++f) {
-      const size_t id = base + i + f * NQ;
-      const double res = fa * A->resQ_in[id] + fdt * r[f];
-      A->resQ[id] = res;
-      U[f] = A->Q_in[id] + fb * res;
-      A->Q[id] = U[f];
+    double res[R][NFLD];
+    matvec(bufA, -2.0, res);
+    for (int r = 0; r < R; ++r) {
+      if (!rv[r]) continue;
+      for (int f = 0; f < NFLD; ++f)
+        bufB[f * NQ + ir[r]] = (DIM == 3) ? res[r][f] * invJ[r] : res[r][f] / J[r];
     }
-    entropy_vars<DIM>(U, gamma, V);
-    for (int f = 0; f < NFLD; ++f) bufA[f * NQ + i] = V[f] * J;
   }
   __syncthreads();
   if (valid) {
-    double r[NFLD];
-    matvec(bufA, 1.0, r);
-    for (int f = 0; f < NFLD; ++f) bufB[f * NQ + i] = (DIM == 3) ? r[f] * invJ : r[f] / J;
+    double res[R][NFLD];
+    matvec(bufB, 1.0, res);
+    for (int r = 0; r < R; ++r) {
+      if (!rv[r]) continue;
+      double U[NFLD], V[NFLD];
+      for (int f = 0; f < NFLD; ++f) {
+        const size_t id = base + ir[r] + f * NQ;
+        const double rq = fa * A->resQ_in[id] + fdt * res[r][f];
+        A->resQ[id] = rq;
+        U[f] = A->Q_in[id] + fb * rq;
+        A->Q[id] = U[f];
+      }
+      entropy_vars<DIM>(U, gamma, V);
+      for (int f = 0; f < NFLD; ++f) bufA[f * NQ + ir[r]] = V[f] * J[r];
+    }
   }
   __syncthreads();
   if (valid) {
-    double r[NFLD];
-    matvec(bufB, 1.0, r);
-    for (int f = 0; f < NFLD; ++f) bufA[f * NQ + i] = r[f];
+    double res[R][NFLD];
+    matvec(bufA, 1.0, res);
+    for (int r = 0; r < R; ++r) {
+      if (!rv[r]) continue;
+      for (int f = 0; f < NFLD; ++f)
+        bufB[f * NQ + ir[r]] = (DIM == 3) ? res[r][f] * invJ[r] : res[r][f] / J[r];
+    }
+  }
+  __syncthreads();
+  if (valid) {
+    double res[R][NFLD];
+    matvec(bufB, 1.0, res);
+    for (int r = 0; r < R; ++r) {
+      if (!rv[r]) continue;
+      for (int f = 0; f < NFLD; ++f) bufA[f * NQ + ir[r]] = res[r][f];
+    }
   }
   __syncthreads();
   // face points, spread over the whole wave: Qf = u(VfPq v~)
@@ -785,11 +908,12 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   }
   // volume points: rhsQ = u(VqPq v~) = u(v~), since VqPq is a projection and
   // v~ = VqPq (...) already lies in its range (round-off apart)
-  if (valid) {
+  for (int r = 0; r < R; ++r) {
+    if (!rv[r]) continue;
     double V[NFLD], U[NFLD];
-    for (int f = 0; f < NFLD; ++f) V[f] = bufA[f * NQ + i];
+    for (int f = 0; f < NFLD; ++f) V[f] = bufA[f * NQ + ir[r]];
     conservative_vars<DIM>(V, gamma, lgm1, U);
-    for (int f = 0; f < NFLD; ++f) A->rhsQ[base + i + f * NQ] = U[f];
+    for (int f = 0; f < NFLD; ++f) A->rhsQ[base + ir[r] + f * NQ] = U[f];
   }
 }
 
@@ -862,13 +986,13 @@ diag_reduce_kernel(int K, const double *__restrict__ aux, double *__restrict__ t
 }
 
 
-template <class S, int TPW>
+template <class S>
 __global__ void __launch_bounds__(WAVE)
 __attribute__((amdgpu_waves_per_eu(vol_waves_min<S>(), vol_waves<S>()))) stage_kernel(StageArgs A) {
   extern __shared__ double smem[];
   __shared__ double s_nref[S::NFACES][S::DIM];
-  stage_body<S, TPW>((const ConstArgs *)&A, A.fa, A.fb, A.Qf_in, A.Qf_out, blockIdx.x * TPW,
-                     threadIdx.x, s_nref, smem);
+  stage_body<S>((const ConstArgs *)&A, A.fa, A.fb, A.Qf_in, A.Qf_out, blockIdx.x * S::TPW,
+                threadIdx.x, s_nref, smem);
 }
 
 #include "wadges_adjoint.hpp"
@@ -921,6 +1045,7 @@ struct Ops {
               double *, hipStream_t);
   int ws_doubles;
   int stage_smem;  // dynamic LDS bytes of one fused-stage workgroup
+  int stage_tpw;   // elements per fused-stage workgroup
 };
 
 }  // namespace wadges
@@ -969,11 +1094,11 @@ template <class S> int vol_impl(const wadges_plan *p, int K, const double *vgeo,
     int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
     if (rc) return rc;
   }
-  constexpr int TPW = WAVE / S::NQ;
+  constexpr int TPW = S::TPW;
   static_assert(TPW >= 1, "Nq must fit in one wave");
   const size_t smem = (size_t)TPW * VolSmem<S>::per_elem * sizeof(double);
   const int blocks = (K + TPW - 1) / TPW;
-  vol_kernel<S, TPW><<<blocks, WAVE, smem, st>>>(K, vgeo, vfgeo, nrJ, nsJ, ntJ, p->ws, Qf, rhsQ,
+  vol_kernel<S><<<blocks, WAVE, smem, st>>>(K, vgeo, vfgeo, nrJ, nsJ, ntJ, p->ws, Qf, rhsQ,
                                                  rhsQf, p->gamma - 1.0,
                                                  1.0 / (2.0 * (p->gamma - 1.0)));
   return hip_check(hipGetLastError());
@@ -1020,7 +1145,7 @@ template <class S> int stage_impl(const wadges_plan *p, int K, const double *vge
   if (K == 0) return WADGES_OK;
   int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
   if (rc) return rc;
-  constexpr int TPW = WAVE / S::NQ;
+  constexpr int TPW = S::TPW;
   const size_t smem = (size_t)TPW * VolSmem<S>::per_elem * sizeof(double);
   const int blocks = (K + TPW - 1) / TPW;
   const double g = p->gamma;
@@ -1028,7 +1153,7 @@ template <class S> int stage_impl(const wadges_plan *p, int K, const double *vge
               p->ws,  VqLq,   VqPq,    VfPq,   Jq,     fa,      fb,   fdt,
               Qf_in,  Qf_out, rhsQ,    resQ,   Q,      rhsQ_in, resQ_in, Q_in,
               g,      g - 1.0, 1.0 / (2.0 * (g - 1.0)), 0.25 * p->tau, std::log(g - 1.0)};
-  stage_kernel<S, TPW><<<blocks, WAVE, smem, st>>>(A);
+  stage_kernel<S><<<blocks, WAVE, smem, st>>>(A);
   return hip_check(hipGetLastError());
 }
 
@@ -1138,7 +1263,7 @@ template <class S> int prep_impl(const double *Drq, const double *Dsq, const dou
 }
 
 template <class S> constexpr int stage_smem_bytes() {
-  return (WAVE / S::NQ) * VolSmem<S>::per_elem * (int)sizeof(double);
+  return S::TPW * VolSmem<S>::per_elem * (int)sizeof(double);
 }
 
 struct Instance {
@@ -1152,7 +1277,7 @@ template <int DIM, int NQ, int NFQ> constexpr Instance make_instance(int N) {
                   Ops{&vol_impl<S>, &surf_impl<S>, &update_impl<S>, &stage_impl<S>,
                       &stage_vjp_impl<S>, &diag_impl<S>,
                       &rhs_vjp_impl<S>, &update_vjp_impl<S>, &proj_vjp_impl<S>, &prep_impl<S>,
-                      S::WS, stage_smem_bytes<S>()}};
+                      S::WS, stage_smem_bytes<S>(), S::TPW}};
 }
 
 // (dim, N, Nq, Nfq): quadrature of degree 2N, the point counts of the reference's

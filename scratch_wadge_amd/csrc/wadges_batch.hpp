@@ -4,7 +4,7 @@
 // Included by wadges.hip at file scope, after the instance table.
 //
 // Device side.  The work of a stage is a list of items (problem, first
-// element); an item is tiles_per_item tiles of floor(64/Nq) elements, the
+// element); an item is tiles_per_item tiles of S::TPW elements, the
 // elements one 64-wide workgroup of the fused stage kernel holds.  A workgroup
 // walks items in a static stride or takes them from an atomic counter, and for
 // each runs stage_body of the item's (Nq, Nfq) instance -- the same device code
@@ -40,7 +40,7 @@ template <class S>
 __device__ __forceinline__ void batch_item(const ConstArgs *A, double fa, double fb,
                                            const double *Qf_in, double *Qf_out, int k0, int tiles,
                                            double (*s_nref)[S::DIM], double *smem) {
-  constexpr int TPW = WAVE / S::NQ;
+  constexpr int TPW = S::TPW;
   for (int t = 0; t < tiles; ++t) {
     const int kfirst = k0 + t * TPW;
     if (kfirst >= A->K) break;          // uniform over the workgroup
@@ -49,7 +49,7 @@ __device__ __forceinline__ void batch_item(const ConstArgs *A, double fa, double
     // across them and multiply the register footprint).
     int lane;
     asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"((int)threadIdx.x));
-    stage_body<S, TPW>(A, fa, fb, Qf_in, Qf_out, kfirst, lane, s_nref, smem);
+    stage_body<S>(A, fa, fb, Qf_in, Qf_out, kfirst, lane, s_nref, smem);
     __syncthreads();                    // LDS is reused by the next tile
   }
 }
@@ -67,9 +67,13 @@ __device__ __forceinline__ void batch_dispatch(ShapeList<Ss...>, int slot, const
 // Persistent form: a workgroup loops over items.  Loop-invariant constants of
 // the transcendental functions would be hoisted out of the loop and held in
 // registers; the occupancy hint makes the register allocator rematerialise them
-// instead (128 VGPRs, four waves per SIMD).
+// instead (2D: 128 VGPRs, four waves per SIMD; 3D: two waves per SIMD, which
+// the LDS of the 3D shapes allows at most, leaving room for the two rows per
+// lane of 3D N = 3).
+template <int DIM> constexpr int batch_waves() { return DIM == 2 ? 4 : 2; }
+
 template <int DIM, class Fam>
-__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(batch_waves<DIM>())))
 batch_stage_persistent(const BatchGroup *__restrict__ groups, const int2 *__restrict__ items,
                        int nitems, int tiles, double fa, double fb, int cur, int *queue) {
   extern __shared__ double smem[];
@@ -320,7 +324,7 @@ int wadges_batch_create(wadges_batch **out, int dim, int ngroups, const wadges_g
       return WADGES_EINVAL;
     if (G.K > 0 && dim == 3 && (!G.ntJ || !G.Dtq)) return WADGES_EINVAL;
     b->slot.push_back(slot);
-    b->tpw.push_back(WAVE / G.Nq);
+    b->tpw.push_back(inst[q]->ops.stage_tpw);
     b->cost.push_back(batch_elem_cost(G.Nq, G.Nfq * (dim + 1)));
     b->smem = std::max(b->smem, (size_t)inst[q]->ops.stage_smem);
     ws_total += (size_t)inst[q]->ops.ws_doubles;
