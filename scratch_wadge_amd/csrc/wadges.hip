@@ -277,10 +277,32 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
     for (int f = 0; f < NFLD; ++f) acc[r][f] = 0.0;
 
   if (valid) {
+    // With R > 1 the operator entries of step t + 1 are loaded while step t
+    // computes (one row per lane has no registers to spare for it).
+    constexpr bool PIPE = R > 1;
     const double *Dfw = ws;
     const double *Dbw = ws + DIM * TVT * NQ;
+    int iw[R];                        // table row (row 0's for rows that do not exist)
+    for (int r = 0; r < R; ++r) iw[r] = rv[r] ? l + r * LPE : 0;
+    double nDf[R][DIM], nDb[R][DIM];
+    auto fetch = [&](int t) {
+      for (int r = 0; r < R; ++r)
+        for (int q = 0; q < DIM; ++q) {
+          nDf[r][q] = Dfw[(q * TVT + t) * NQ + iw[r]];
+          nDb[r][q] = Dbw[(q * TVT + t) * NQ + iw[r]];
+        }
+    };
+    if (PIPE) fetch(0);
     for (int t = 0; t < TVT; ++t) {
       if (2 * t > NQ) break;
+      if (!PIPE) fetch(t);
+      double Df[R][DIM], Db[R][DIM];
+      for (int r = 0; r < R; ++r)
+        for (int q = 0; q < DIM; ++q) {
+          Df[r][q] = nDf[r][q];
+          Db[r][q] = nDb[r][q];
+        }
+      if (PIPE && t + 1 < TVT && 2 * (t + 1) <= NQ) fetch(t + 1);
       for (int r = 0; r < R; ++r) {
         if (!rv[r]) continue;
         const int i = l + r * LPE;
@@ -293,19 +315,15 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
         const PairFlux<DIM> P = pair_flux<DIM>(me[r], pj, inv2gm1);
         double Dx[DIM];
         for (int d = 0; d < DIM; ++d) Dx[d] = 0.0;
-        for (int q = 0; q < DIM; ++q) {
-          const double Dr = Dfw[(q * TVT + t) * NQ + i];
-          for (int d = 0; d < DIM; ++d) Dx[d] = Dx[d] + Dr * Ga[q * DIM + d];
-        }
+        for (int q = 0; q < DIM; ++q)
+          for (int d = 0; d < DIM; ++d) Dx[d] = Dx[d] + Df[r][q] * Ga[q * DIM + d];
         double c[NFLD];
         flux_dot<DIM>(P, Dx, c);
         for (int f = 0; f < NFLD; ++f) acc[r][f] += c[f];
         if (t > 0) {
           for (int d = 0; d < DIM; ++d) Dx[d] = 0.0;
-          for (int q = 0; q < DIM; ++q) {
-            const double Dr = Dbw[(q * TVT + t) * NQ + i];
-            for (int d = 0; d < DIM; ++d) Dx[d] = Dx[d] + Dr * Ga[q * DIM + d];
-          }
+          for (int q = 0; q < DIM; ++q)
+            for (int d = 0; d < DIM; ++d) Dx[d] = Dx[d] + Db[r][q] * Ga[q * DIM + d];
           flux_dot<DIM>(P, Dx, c);
           for (int f = 0; f < NFLD; ++f) lds_add(&accV[f * NQ + j], c[f]);
         }
@@ -354,34 +372,53 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
     if (valid) {
       if constexpr (R > 1) {
         // part h is face h; at step m lane l visits its point (l + m) mod NFQ,
-        // and the one record read serves all rows of the lane
+        // and the one record read serves all rows of the lane.  The partner
+        // record and the weights of step m + 1 are loaded while step m computes.
         const double *W1 = ws + S::WS_D + S::WS_W;
         const double *W2 = W1 + NFN * NQ;
         double a[R][DIM];
-        for (int r = 0; r < R; ++r)
+        int iw[R];                        // weight row (row 0's for rows that do not exist)
+        for (int r = 0; r < R; ++r) {
+          iw[r] = rv[r] ? l + r * LPE : 0;
           if (rv[r]) own_normal(r, h, a[r]);
-        for (int m = 0; m < NFQ; ++m) {
+        }
+        auto fetch = [&](int m, Node<DIM> &pf, double (&b)[DIM], double (&w1)[R], double (&w2)[R]) {
           int q = l + m;
           if (q >= NFQ) q -= NFQ;
-          const Node<DIM> pf = load_node<DIM>(frec, NH, q);
-          double b[DIM];
+          pf = load_node<DIM>(frec, NH, q);
           for (int d = 0; d < DIM; ++d) b[d] = frec[(VS + d) * NH + q];
-          double sF[NFLD];
-          for (int f = 0; f < NFLD; ++f) sF[f] = 0.0;
           const int t = h * NFQ + m;
           for (int r = 0; r < R; ++r) {
+            w1[r] = W1[t * NQ + iw[r]];
+            w2[r] = W2[t * NQ + iw[r]];
+          }
+        };
+        Node<DIM> nf;
+        double nb[DIM], nw1[R], nw2[R];
+        fetch(0, nf, nb, nw1, nw2);
+        for (int m = 0; m < NFQ; ++m) {
+          const Node<DIM> pf = nf;
+          double b[DIM], w1[R], w2[R];
+          for (int d = 0; d < DIM; ++d) b[d] = nb[d];
+          for (int r = 0; r < R; ++r) {
+            w1[r] = nw1[r];
+            w2[r] = nw2[r];
+          }
+          if (m + 1 < NFQ) fetch(m + 1, nf, nb, nw1, nw2);
+          int q = l + m;
+          if (q >= NFQ) q -= NFQ;
+          double sF[NFLD];
+          for (int f = 0; f < NFLD; ++f) sF[f] = 0.0;
+          for (int r = 0; r < R; ++r) {
             if (!rv[r]) continue;
-            const int i = l + r * LPE;
             double n[DIM];
             for (int d = 0; d < DIM; ++d) n[d] = a[r][d] + b[d];
             const PairFlux<DIM> P = pair_flux<DIM>(me[r], pf, inv2gm1);
             double Fn[NFLD];
             flux_dot<DIM>(P, n, Fn);
-            const double w1 = W1[t * NQ + i];
-            const double w2 = W2[t * NQ + i];
             for (int f = 0; f < NFLD; ++f) {
-              acc[r][f] += w1 * Fn[f];
-              sF[f] += w2 * Fn[f];
+              acc[r][f] += w1[r] * Fn[f];
+              sF[f] += w2[r] * Fn[f];
             }
           }
           for (int f = 0; f < NFLD; ++f) lds_add(&accF[f * NFN + h * NFQ + q], sF[f]);
