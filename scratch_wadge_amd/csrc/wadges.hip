@@ -55,8 +55,19 @@ template <int A, int B> struct Max { static constexpr int value = A > B ? A : B;
 // (Nq = 23) packs 5 elements of 12 lanes (60 lanes, 23 of 24 rows used)
 // instead of 2 elements of 23 lanes (46 lanes).  R > 1 requires the lanes of an
 // element to fit on one face (LPE <= Nfq): the volume-face block then walks
-// the faces one at a time.
+// the faces one at a time.  3D N = 4 (Nq = 44, Nfq = 16) keeps one row per lane:
+// three rows (4 elements of 15 lanes) need more registers than the two waves
+// per SIMD its LDS allows, and at one wave per SIMD the volume kernel took
+// 4.52 ms instead of 3.70 ms at K = 196,608.
 constexpr int vol_rows(int dim, int nq, int nfq) { return (dim == 3 && nq == 23 && nfq == 12) ? 2 : 1; }
+
+// p[idx] for a uniform base p and a per-lane index below 2^29: a 32-bit unsigned
+// byte offset lets the compiler address through the 64-bit base in scalar
+// registers plus a 32-bit lane offset, instead of 64-bit address arithmetic
+// per load.
+__device__ __forceinline__ double ld_off(const double *p, unsigned idx) {
+  return *(const double *)((const char *)p + (idx * 8u));
+}
 
 // Compile-time shape of one (dim, Nq, Nfq) instance.
 template <int DIM_, int NQ_, int NFQ_> struct Shape {
@@ -288,8 +299,8 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
     auto fetch = [&](int t) {
       for (int r = 0; r < R; ++r)
         for (int q = 0; q < DIM; ++q) {
-          nDf[r][q] = Dfw[(q * TVT + t) * NQ + iw[r]];
-          nDb[r][q] = Dbw[(q * TVT + t) * NQ + iw[r]];
+          nDf[r][q] = ld_off(Dfw, (unsigned)((q * TVT + t) * NQ + iw[r]));
+          nDb[r][q] = ld_off(Dbw, (unsigned)((q * TVT + t) * NQ + iw[r]));
         }
     };
     if (PIPE) fetch(0);
@@ -389,8 +400,8 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
           for (int d = 0; d < DIM; ++d) b[d] = frec[(VS + d) * NH + q];
           const int t = h * NFQ + m;
           for (int r = 0; r < R; ++r) {
-            w1[r] = W1[t * NQ + iw[r]];
-            w2[r] = W2[t * NQ + iw[r]];
+            w1[r] = ld_off(W1, (unsigned)(t * NQ + iw[r]));
+            w2[r] = ld_off(W2, (unsigned)(t * NQ + iw[r]));
           }
         };
         Node<DIM> nf;
@@ -435,8 +446,8 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
           const PairFlux<DIM> P = pair_flux<DIM>(me[0], pf, inv2gm1);
           double Fn[NFLD];
           flux_dot<DIM>(P, n, Fn);
-          const double w1 = W1[t * NQ + i];
-          const double w2 = W2[t * NQ + i];
+          const double w1 = ld_off(W1, (unsigned)(t * NQ + i));
+          const double w2 = ld_off(W2, (unsigned)(t * NQ + i));
           for (int f = 0; f < NFLD; ++f) {
             acc[0][f] += w1 * Fn[f];
             lds_add(&accF[f * NFN + fn], w2 * Fn[f]);
@@ -584,7 +595,7 @@ surf_kernel(int K, const double *__restrict__ fgeo, const uint32_t *__restrict__
       for (int f = 0; f < NFLD; ++f) val[f] = 0.0;
 #pragma unroll 2
       for (int j = 0; j < NFN; ++j) {
-        const double Lij = VqLq[row + j * NQ];
+        const double Lij = ld_off(VqLq, (unsigned)(row + j * NQ));
         for (int f = 0; f < NFLD; ++f) val[f] += Lij * sfS[e][f][j];
       }
       double *dst = rhsQ + (size_t)k * NQ * NFLD;
@@ -819,7 +830,7 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
       double x[NFLD];
       for (int f = 0; f < NFLD; ++f) x[f] = accF[f * NFN + j];
       for (int r = 0; r < R; ++r) {
-        const double L = A->VqLq[ir[r] + j * NQ];
+        const double L = ld_off(A->VqLq, (unsigned)(ir[r] + j * NQ));
         for (int f = 0; f < NFLD; ++f) val[r][f] += L * x[f];
       }
     }
@@ -851,7 +862,7 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   if constexpr (ROW_IN_REGS) {
     if (valid)
       for (int r = 0; r < R; ++r)
-        for (int j = 0; j < NQ; ++j) Prow[r][j] = A->VqPq[ir[r] + j * NQ];
+        for (int j = 0; j < NQ; ++j) Prow[r][j] = ld_off(A->VqPq, (unsigned)(ir[r] + j * NQ));
   }
   auto matvec = [&](const double *in, double scale, double (&res)[R][NFLD]) {
     for (int r = 0; r < R; ++r)
@@ -872,7 +883,7 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
         double x[NFLD];
         for (int f = 0; f < NFLD; ++f) x[f] = in[f * NQ + j];
         for (int r = 0; r < R; ++r) {
-          const double P = scale * A->VqPq[ir[r] + j * NQ];
+          const double P = scale * ld_off(A->VqPq, (unsigned)(ir[r] + j * NQ));
           for (int f = 0; f < NFLD; ++f) res[r][f] += P * x[f];
         }
       }
@@ -936,7 +947,7 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
     for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
 #pragma unroll 8
     for (int j = 0; j < NQ; ++j) {
-      const double P = A->VfPq[fn + j * NFN];
+      const double P = ld_off(A->VfPq, (unsigned)(fn + j * NFN));
       for (int f = 0; f < NFLD; ++f) V[f] += P * bufAt[f * NQ + j];
     }
     conservative_vars<DIM>(V, gamma, lgm1, U);

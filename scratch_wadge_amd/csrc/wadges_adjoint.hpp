@@ -46,7 +46,7 @@ sbar_kernel(int K, const double *__restrict__ VqLq, const double *__restrict__ R
     for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
 #pragma unroll 4
     for (int i = 0; i < NQ; ++i) {
-      const double L = VqLq[i + lane * NQ];
+      const double L = ld_off(VqLq, (unsigned)(i + lane * NQ));
       for (int f = 0; f < NFLD; ++f) v[f] += L * sR[e][f][i];
     }
     for (int f = 0; f < NFLD; ++f) sbar[(size_t)k * NFN * NFLD + f * NFN + lane] = v[f];
@@ -143,8 +143,8 @@ rhs_vjp_kernel(int K, const double *__restrict__ vgeo, const double *__restrict_
       double Dxf[DIM], Dxb[DIM];
       for (int d = 0; d < DIM; ++d) Dxf[d] = Dxb[d] = 0.0;
       for (int r = 0; r < DIM; ++r) {
-        const double a = Dfw[(r * TVT + t) * NQ + i];
-        const double b = Dbw[(r * TVT + t) * NQ + i];
+        const double a = ld_off(Dfw, (unsigned)((r * TVT + t) * NQ + i));
+        const double b = ld_off(Dbw, (unsigned)((r * TVT + t) * NQ + i));
         for (int d = 0; d < DIM; ++d) {
           Dxf[d] = Dxf[d] + a * Ga[r * DIM + d];
           Dxb[d] = Dxb[d] + b * Ga[r * DIM + d];
@@ -201,8 +201,8 @@ rhs_vjp_kernel(int K, const double *__restrict__ vgeo, const double *__restrict_
       const Node<DIM> pf = load_node<DIM>(frec, NFN, fn);
       double n[DIM];
       for (int d = 0; d < DIM; ++d) n[d] = a[d] + frec[(VS + d) * NFN + fn];
-      const double w1 = W1[t * NQ + i];
-      const double w2 = W2[t * NQ + i];
+      const double w1 = ld_off(W1, (unsigned)(t * NQ + i));
+      const double w2 = ld_off(W2, (unsigned)(t * NQ + i));
       double c[NFLD];
       for (int f = 0; f < NFLD; ++f) c[f] = w1 * Rbi[f] + w2 * sbF[f * NFN + fn];
       PrimBar<DIM> gf;
@@ -502,7 +502,7 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
       for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
 #pragma unroll 4
       for (int j = 0; j < NQ; ++j) {
-        const double L = VqLq[j + fn * NQ];
+        const double L = ld_off(VqLq, (unsigned)(j + fn * NQ));
         for (int f = 0; f < NFLD; ++f) v[f] += L * bufA[e][f][j];
       }
       for (int f = 0; f < NFLD; ++f) sbar[basef + f * NFN + fn] = v[f];
@@ -546,7 +546,7 @@ proj_vjp_kernel(int K, const double *__restrict__ VqPq, const double *__restrict
     double V[NFLD], Ub[NFLD];
     for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
     for (int j = 0; j < NQ; ++j) {
-      const double P = VqPq[i + j * NQ];
+      const double P = ld_off(VqPq, (unsigned)(i + j * NQ));
       for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[e][f][j];
     }
     for (int f = 0; f < NFLD; ++f) Ub[f] = uqb[base + i + f * NQ];
@@ -556,7 +556,7 @@ proj_vjp_kernel(int K, const double *__restrict__ VqPq, const double *__restrict
       if (fn >= NFN) continue;
       for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
       for (int j = 0; j < NQ; ++j) {
-        const double P = VfPq[fn + j * NFN];
+        const double P = ld_off(VfPq, (unsigned)(fn + j * NFN));
         for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[e][f][j];
       }
       for (int f = 0; f < NFLD; ++f) Ub[f] = ufb[basef + fn + f * NFN];
@@ -577,11 +577,11 @@ proj_vjp_kernel(int K, const double *__restrict__ VqPq, const double *__restrict
     double vb[NFLD], Ub[NFLD];
     for (int f = 0; f < NFLD; ++f) vb[f] = 0.0;
     for (int j = 0; j < NQ; ++j) {
-      const double P = VqPq[j + i * NQ];
+      const double P = ld_off(VqPq, (unsigned)(j + i * NQ));
       for (int f = 0; f < NFLD; ++f) vb[f] += P * bufB[e][f][j];
     }
     for (int fn = 0; fn < NFN; ++fn) {
-      const double P = VfPq[fn + i * NFN];
+      const double P = ld_off(VfPq, (unsigned)(fn + i * NFN));
       for (int f = 0; f < NFLD; ++f) vb[f] += P * bufF[e][f][fn];
     }
     entropy_vars_vjp<DIM>(U, gamma, vb, Ub);
