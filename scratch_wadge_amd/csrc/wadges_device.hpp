@@ -91,8 +91,25 @@ template <int DIM> __device__ __forceinline__ double logmean_series(double u) {
     return 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0))));
 }
 
-// inv2gm1 = 1 / (2 (gamma - 1)).  Both log-mean branches are evaluated and
-// selected without branching (lanes of a wave take different branches).
+// 2 * logmean_series(u), bit for bit: scaling by 2 is exact, so Horner with
+// doubled constants rounds every step to twice the value (one multiply fewer).
+template <int DIM> __device__ __forceinline__ double logmean_series2(double u) {
+  if constexpr (DIM == 3)
+    return 2.0 + u * (2.0 / 3.0 + u * (2.0 / 5.0 + u * (2.0 / 7.0)));
+  else
+    return 2.0 + u * (2.0 / 3.0 + u * (2.0 / 5.0 + u * (2.0 / 7.0 + u * (2.0 / 9.0))));
+}
+
+// True when p holds on every active lane of the wave (a scalar test).
+__device__ __forceinline__ bool wave_all(bool p) {
+  return __builtin_amdgcn_ballot_w64(p) == __builtin_amdgcn_read_exec();
+}
+
+// inv2gm1 = 1 / (2 (gamma - 1)).  Lanes of a wave may take different log-mean
+// branches: both are evaluated and selected per lane, except when every active
+// lane takes the series branch for both log means (smooth states: neighbouring
+// nodes differ by far less than the 1 % that the series threshold allows),
+// where the selection is skipped.  Both ways give the same bits.
 template <int DIM>
 __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Node<DIM> &b,
                                                    double inv2gm1) {
@@ -104,16 +121,19 @@ __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Nod
   const double fr = dr * sb * q1;
   const double fb = db * sr * q1;
   const double ur = fr * fr, ub = fb * fb;
-  const double polyR = 2.0 * logmean_series<DIM>(ur);
-  const double polyB = 2.0 * logmean_series<DIM>(ub);
+  const double polyR = logmean_series2<DIM>(ur);
+  const double polyB = logmean_series2<DIM>(ub);
   const double ldR = a.lrho - b.lrho;
   const double ldB = a.lbeta - b.lbeta;
   const bool serR = ur < TH, serB = ub < TH;
   // logmean = num / den:  series (aL+aR) / (2 F(u)),  log (aL-aR) / (ln aL - ln aR)
-  const double numR = serR ? sr : dr;
-  const double denR = serR ? polyR : ldR;
-  const double numB = serB ? sb : db;
-  const double denB = serB ? polyB : ldB;
+  double numR = sr, denR = polyR, numB = sb, denB = polyB;
+  if (!wave_all(serR && serB)) {
+    numR = serR ? sr : dr;
+    denR = serR ? polyR : ldR;
+    numB = serB ? sb : db;
+    denB = serB ? polyB : ldB;
+  }
   // rholog = numR/denR, rholog/betalog = numR denB/(denR numB), pa = (sr/2)/sb
   const double q = rcp(denR * numB * sb);
   const double A = numB * q;
