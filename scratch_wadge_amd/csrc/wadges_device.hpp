@@ -306,15 +306,51 @@ __device__ __forceinline__ void flux_vjp(const Node<DIM> &a, const Node<DIM> &b,
 }
 
 // Same for a rank-one cotangent Gb[d][fld] = n[d] c[fld] (the contracted
-// volume-face and surface couplings).
+// volume-face and surface couplings).  With Un = n.va and cm = sum_m c[1+m] va_m
+// the contractions of flux_vjp reduce to A = c[0] Un, B = Un cm,
+// C = c[last] Un, tr = sum_d n_d c[1+d] and, per direction,
+// t_d = n_d (c[0] + cm) + c[1+d] Un: O(DIM + NFLD) work instead of O(DIM^2 NFLD).
 template <int DIM>
 __device__ __forceinline__ void flux_dot_vjp(const Node<DIM> &a, const Node<DIM> &b,
                                              const double (&n)[DIM], const double (&c)[DIM + 2],
                                              double inv2gm1, PrimBar<DIM> &ga, PrimBar<DIM> &gb) {
-  double Gb[DIM][DIM + 2];
-  for (int d = 0; d < DIM; ++d)
-    for (int f = 0; f < DIM + 2; ++f) Gb[d][f] = n[d] * c[f];
-  flux_vjp<DIM>(a, b, Gb, inv2gm1, ga, gb);
+  constexpr int NF = DIM + 2;
+  double rl, rla, rlb, bl, bla, blb;
+  logmean_partials<DIM>(a.rho, b.rho, a.lrho, b.lrho, rl, rla, rlb);
+  logmean_partials<DIM>(a.beta, b.beta, a.lbeta, b.lbeta, bl, bla, blb);
+  const double sr = a.rho + b.rho, sb = a.beta + b.beta;
+  const double isb = rcp(sb);
+  const double pa = 0.5 * sr * isb;
+  double va[DIM], vv = 0.0;
+  for (int d = 0; d < DIM; ++d) {
+    va[d] = a.hv[d] + b.hv[d];
+    vv = vv + va[d] * va[d];
+  }
+  const double vnavg = 2.0 * vv - (a.hq + b.hq);
+  const double ibl = rcp(bl);
+  const double f4 = rl * inv2gm1 * ibl + pa + 0.5 * rl * vnavg;
+  double Un = n[0] * va[0], cm = c[1] * va[0], tr = n[0] * c[1];
+  for (int d = 1; d < DIM; ++d) {
+    Un = Un + n[d] * va[d];
+    cm = cm + c[1 + d] * va[d];
+    tr = tr + n[d] * c[1 + d];
+  }
+  const double C = c[NF - 1] * Un;
+  const double L_rl = c[0] * Un + Un * cm + C * (inv2gm1 * ibl + 0.5 * vnavg);
+  const double L_bl = -C * rl * inv2gm1 * ibl * ibl;
+  const double L_pa = tr + C;
+  const double c0m = c[0] + cm;
+  for (int d = 0; d < DIM; ++d) {
+    const double t = n[d] * c0m + c[1 + d] * Un;
+    const double L_va = rl * t + f4 * n[d] * c[NF - 1] + C * 2.0 * rl * va[d];
+    const double ua = 2.0 * a.hv[d], ub = 2.0 * b.hv[d];
+    ga.u[d] += 0.5 * L_va - 0.5 * C * rl * ua;
+    gb.u[d] += 0.5 * L_va - 0.5 * C * rl * ub;
+  }
+  ga.rho += L_rl * rla + L_pa * 0.5 * isb;
+  gb.rho += L_rl * rlb + L_pa * 0.5 * isb;
+  ga.beta += L_bl * bla - L_pa * pa * isb;
+  gb.beta += L_bl * blb - L_pa * pa * isb;
 }
 
 // Primitive cotangent -> conservative cotangent of one node:
