@@ -150,13 +150,23 @@ rhs_vjp_kernel(int K, const double *__restrict__ vgeo, const double *__restrict_
           Dxb[d] = Dxb[d] + b * Ga[r * DIM + d];
         }
       }
-      double Gb[DIM][NFLD];
-      for (int d = 0; d < DIM; ++d)
-        for (int f = 0; f < NFLD; ++f)
-          Gb[d][f] = Rbi[f] * Dxf[d] + (t > 0 ? RbV[f * NQ + j] * Dxb[d] : 0.0);
+      // cotangent Rb_i (x) Dxf + Rb_j (x) Dxb (the partner row only off the diagonal)
       PrimBar<DIM> gj;
       primbar_zero<DIM>(gj);
-      flux_vjp<DIM>(me, pj, Gb, inv2gm1, gI, gj);
+      if (t > 0) {
+        double nk[2][DIM], ck[2][NFLD];
+        for (int d = 0; d < DIM; ++d) {
+          nk[0][d] = Dxf[d];
+          nk[1][d] = Dxb[d];
+        }
+        for (int f = 0; f < NFLD; ++f) {
+          ck[0][f] = Rbi[f];
+          ck[1][f] = RbV[f * NQ + j];
+        }
+        flux_dot_vjp_k<DIM, 2>(me, pj, nk, ck, inv2gm1, gI, gj);
+      } else {
+        flux_dot_vjp<DIM>(me, pj, Dxf, Rbi, inv2gm1, gI, gj);
+      }
       if (t == 0) {
         gI.rho += gj.rho;
         for (int d = 0; d < DIM; ++d) gI.u[d] += gj.u[d];

@@ -258,11 +258,21 @@ __device__ __forceinline__ void logmean_partials(double a, double b, double la, 
 }
 
 // VJP of L = sum_d sum_fld Gb[d][fld] F_d[fld](a, b) with respect to the
-// primitive quantities of both nodes; accumulates into ga, gb.
-template <int DIM>
-__device__ __forceinline__ void flux_vjp(const Node<DIM> &a, const Node<DIM> &b,
-                                         const double (&Gb)[DIM][DIM + 2], double inv2gm1,
-                                         PrimBar<DIM> &ga, PrimBar<DIM> &gb) {
+// primitive quantities of both nodes, accumulated into ga, gb, for a cotangent
+// of rank NT, Gb[d][fld] = sum_k n_k[d] c_k[fld] (rank one: the contracted
+// volume-face and surface couplings; rank two: a volume pair's own row and its
+// partner row).  L depends on the flux through A = sum_d Gb[d][0] va_d,
+// B = sum_dm Gb[d][1+m] va_d va_m, C = sum_d Gb[d][last] va_d and
+// tr = sum_d Gb[d][1+d]; with Un_k = n_k.va and cm_k = sum_m c_k[1+m] va_m these
+// are A = sum c_k[0] Un_k, B = sum Un_k cm_k, C = sum c_k[last] Un_k,
+// tr = sum_k sum_d n_k[d] c_k[1+d], and the velocity derivative needs, per
+// direction, t_d = sum_k n_k[d] (c_k[0] + cm_k) + c_k[1+d] Un_k:
+// O(NT (DIM + NFLD)) work instead of the O(DIM^2 NFLD) of a dense Gb.
+template <int DIM, int NT>
+__device__ __forceinline__ void flux_dot_vjp_k(const Node<DIM> &a, const Node<DIM> &b,
+                                               const double (&n)[NT][DIM],
+                                               const double (&c)[NT][DIM + 2], double inv2gm1,
+                                               PrimBar<DIM> &ga, PrimBar<DIM> &gb) {
   constexpr int NF = DIM + 2;
   double rl, rla, rlb, bl, bla, blb;
   logmean_partials<DIM>(a.rho, b.rho, a.lrho, b.lrho, rl, rla, rlb);
@@ -278,23 +288,30 @@ __device__ __forceinline__ void flux_vjp(const Node<DIM> &a, const Node<DIM> &b,
   const double vnavg = 2.0 * vv - (a.hq + b.hq);
   const double ibl = rcp(bl);
   const double f4 = rl * inv2gm1 * ibl + pa + 0.5 * rl * vnavg;
-  // contractions of the cotangent with the flux structure
-  double A = 0.0, B = 0.0, tr = 0.0, C = 0.0;
-  for (int d = 0; d < DIM; ++d) {
-    A = A + Gb[d][0] * va[d];
-    C = C + Gb[d][NF - 1] * va[d];
-    tr = tr + Gb[d][1 + d];
-    for (int m = 0; m < DIM; ++m) B = B + Gb[d][1 + m] * va[d] * va[m];
+  double Un[NT], c0m[NT], A = 0.0, B = 0.0, C = 0.0, tr = 0.0;
+  for (int k = 0; k < NT; ++k) {
+    double u = n[k][0] * va[0], cm = c[k][1] * va[0];
+    for (int d = 1; d < DIM; ++d) {
+      u = u + n[k][d] * va[d];
+      cm = cm + c[k][1 + d] * va[d];
+    }
+    for (int d = 0; d < DIM; ++d) tr = tr + n[k][d] * c[k][1 + d];
+    Un[k] = u;
+    c0m[k] = c[k][0] + cm;
+    A = A + c[k][0] * u;
+    B = B + u * cm;
+    C = C + c[k][NF - 1] * u;
   }
   const double L_rl = A + B + C * (inv2gm1 * ibl + 0.5 * vnavg);
   const double L_bl = -C * rl * inv2gm1 * ibl * ibl;
   const double L_pa = tr + C;
-  // d/d(va_d): the va enter F directly and through vnavg = 2|va|^2 - ...
   for (int d = 0; d < DIM; ++d) {
-    double t = Gb[d][0];
-    for (int m = 0; m < DIM; ++m) t = t + (Gb[d][1 + m] + Gb[m][1 + d]) * va[m];
-    const double L_va = rl * t + f4 * Gb[d][NF - 1] + C * 2.0 * rl * va[d];
-    // va = (u_a + u_b)/2; vnavg also holds -(|u_a|^2 + |u_b|^2)/2
+    double t = 0.0, e = 0.0;
+    for (int k = 0; k < NT; ++k) {
+      t = t + n[k][d] * c0m[k] + c[k][1 + d] * Un[k];
+      e = e + n[k][d] * c[k][NF - 1];
+    }
+    const double L_va = rl * t + f4 * e + C * 2.0 * rl * va[d];
     const double ua = 2.0 * a.hv[d], ub = 2.0 * b.hv[d];
     ga.u[d] += 0.5 * L_va - 0.5 * C * rl * ua;
     gb.u[d] += 0.5 * L_va - 0.5 * C * rl * ub;
@@ -305,52 +322,14 @@ __device__ __forceinline__ void flux_vjp(const Node<DIM> &a, const Node<DIM> &b,
   gb.beta += L_bl * blb - L_pa * pa * isb;
 }
 
-// Same for a rank-one cotangent Gb[d][fld] = n[d] c[fld] (the contracted
-// volume-face and surface couplings).  With Un = n.va and cm = sum_m c[1+m] va_m
-// the contractions of flux_vjp reduce to A = c[0] Un, B = Un cm,
-// C = c[last] Un, tr = sum_d n_d c[1+d] and, per direction,
-// t_d = n_d (c[0] + cm) + c[1+d] Un: O(DIM + NFLD) work instead of O(DIM^2 NFLD).
 template <int DIM>
 __device__ __forceinline__ void flux_dot_vjp(const Node<DIM> &a, const Node<DIM> &b,
                                              const double (&n)[DIM], const double (&c)[DIM + 2],
                                              double inv2gm1, PrimBar<DIM> &ga, PrimBar<DIM> &gb) {
-  constexpr int NF = DIM + 2;
-  double rl, rla, rlb, bl, bla, blb;
-  logmean_partials<DIM>(a.rho, b.rho, a.lrho, b.lrho, rl, rla, rlb);
-  logmean_partials<DIM>(a.beta, b.beta, a.lbeta, b.lbeta, bl, bla, blb);
-  const double sr = a.rho + b.rho, sb = a.beta + b.beta;
-  const double isb = rcp(sb);
-  const double pa = 0.5 * sr * isb;
-  double va[DIM], vv = 0.0;
-  for (int d = 0; d < DIM; ++d) {
-    va[d] = a.hv[d] + b.hv[d];
-    vv = vv + va[d] * va[d];
-  }
-  const double vnavg = 2.0 * vv - (a.hq + b.hq);
-  const double ibl = rcp(bl);
-  const double f4 = rl * inv2gm1 * ibl + pa + 0.5 * rl * vnavg;
-  double Un = n[0] * va[0], cm = c[1] * va[0], tr = n[0] * c[1];
-  for (int d = 1; d < DIM; ++d) {
-    Un = Un + n[d] * va[d];
-    cm = cm + c[1 + d] * va[d];
-    tr = tr + n[d] * c[1 + d];
-  }
-  const double C = c[NF - 1] * Un;
-  const double L_rl = c[0] * Un + Un * cm + C * (inv2gm1 * ibl + 0.5 * vnavg);
-  const double L_bl = -C * rl * inv2gm1 * ibl * ibl;
-  const double L_pa = tr + C;
-  const double c0m = c[0] + cm;
-  for (int d = 0; d < DIM; ++d) {
-    const double t = n[d] * c0m + c[1 + d] * Un;
-    const double L_va = rl * t + f4 * n[d] * c[NF - 1] + C * 2.0 * rl * va[d];
-    const double ua = 2.0 * a.hv[d], ub = 2.0 * b.hv[d];
-    ga.u[d] += 0.5 * L_va - 0.5 * C * rl * ua;
-    gb.u[d] += 0.5 * L_va - 0.5 * C * rl * ub;
-  }
-  ga.rho += L_rl * rla + L_pa * 0.5 * isb;
-  gb.rho += L_rl * rlb + L_pa * 0.5 * isb;
-  ga.beta += L_bl * bla - L_pa * pa * isb;
-  gb.beta += L_bl * blb - L_pa * pa * isb;
+  double nk[1][DIM], ck[1][DIM + 2];
+  for (int d = 0; d < DIM; ++d) nk[0][d] = n[d];
+  for (int f = 0; f < DIM + 2; ++f) ck[0][f] = c[f];
+  flux_dot_vjp_k<DIM, 1>(a, b, nk, ck, inv2gm1, ga, gb);
 }
 
 // Primitive cotangent -> conservative cotangent of one node:
