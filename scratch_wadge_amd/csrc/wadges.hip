@@ -782,6 +782,25 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   vol_body<S>(valid, l, A->rhsQ_in + base, A->vgeo + kk * NQ * NG, s_nref, A->ws, accV, gm1,
               inv2gm1, acc, wc);
 
+  // The update's per-row inputs.  With R > 1 they are loaded now, so that their
+  // HBM latency passes under the surface and lift phases (the other shapes
+  // have no registers to spare for them and load them where they are used).
+  constexpr bool EARLY = R > 1;
+  int ir[R];
+  for (int r = 0; r < R; ++r) ir[r] = rv[r] ? l + r * LPE : 0;
+  double Qin[EARLY ? R : 1][NFLD], resin[EARLY ? R : 1][NFLD], J[R];
+  for (int r = 0; r < R; ++r) {
+    J[r] = 1.0;
+    if (EARLY && rv[r]) {
+      J[r] = A->Jq[kk * NQ + ir[r]];
+      for (int f = 0; f < NFLD; ++f) {
+        const size_t id = base + ir[r] + f * NQ;
+        Qin[r][f] = A->Q_in[id];
+        resin[r][f] = A->resQ_in[id];
+      }
+    }
+  }
+
   // ---------------- surface flux + LF penalty, in place over the face rows.
   // The face points of all TPW elements are spread over the whole wave.
   for (int p = lane; p < TPW * NFN; p += WAVE) {
@@ -818,8 +837,6 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   __syncthreads();
   // ---------------- RHS rows = volume part + lift of the face terms
   // (rows that do not exist read row 0's operator entries and are never stored)
-  int ir[R];
-  for (int r = 0; r < R; ++r) ir[r] = rv[r] ? l + r * LPE : 0;
   double rhs[R][NFLD];
   if (valid) {
     double val[R][NFLD];
@@ -842,12 +859,11 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   // ---------------- update (same sequence as update_kernel)
   double *bufA = accV;                // [NFLD][NQ]
   double *bufB = accV + NFLD * NQ;    // [NFLD][NQ]
-  double J[R], invJ[R];
+  double invJ[R];
   for (int r = 0; r < R; ++r) {
-    J[r] = 1.0;
     invJ[r] = 1.0;
     if (rv[r]) {
-      J[r] = A->Jq[kk * NQ + ir[r]];
+      if (!EARLY) J[r] = A->Jq[kk * NQ + ir[r]];
       invJ[r] = 1.0 / J[r];
       for (int f = 0; f < NFLD; ++f) bufA[f * NQ + ir[r]] = rhs[r][f];
     }
@@ -907,9 +923,9 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
       double U[NFLD], V[NFLD];
       for (int f = 0; f < NFLD; ++f) {
         const size_t id = base + ir[r] + f * NQ;
-        const double rq = fa * A->resQ_in[id] + fdt * res[r][f];
+        const double rq = fa * (EARLY ? resin[r][f] : A->resQ_in[id]) + fdt * res[r][f];
         A->resQ[id] = rq;
-        U[f] = A->Q_in[id] + fb * rq;
+        U[f] = (EARLY ? Qin[r][f] : A->Q_in[id]) + fb * rq;
         A->Q[id] = U[f];
       }
       entropy_vars<DIM>(U, gamma, V);

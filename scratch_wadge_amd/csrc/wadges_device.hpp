@@ -106,10 +106,12 @@ __device__ __forceinline__ bool wave_all(bool p) {
 }
 
 // inv2gm1 = 1 / (2 (gamma - 1)).  Lanes of a wave may take different log-mean
-// branches: both are evaluated and selected per lane, except when every active
-// lane takes the series branch for both log means (smooth states: neighbouring
-// nodes differ by far less than the 1 % that the series threshold allows),
-// where the selection is skipped.  Both ways give the same bits.
+// branches: both are evaluated and selected per lane, except in 3D when every
+// active lane takes the series branch for both log means (smooth states:
+// neighbouring nodes differ by far less than the 1 % that the series threshold
+// allows), where the selection is skipped.  Both ways give the same bits.  (In
+// 2D the kernels aim at four or more waves per SIMD, whose register budget
+// cannot hold both branches' values across the test: it spilled.)
 template <int DIM>
 __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Node<DIM> &b,
                                                    double inv2gm1) {
@@ -128,7 +130,7 @@ __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Nod
   const bool serR = ur < TH, serB = ub < TH;
   // logmean = num / den:  series (aL+aR) / (2 F(u)),  log (aL-aR) / (ln aL - ln aR)
   double numR = sr, denR = polyR, numB = sb, denB = polyB;
-  if (!wave_all(serR && serB)) {
+  if (DIM == 2 || !wave_all(serR && serB)) {
     numR = serR ? sr : dr;
     denR = serR ? polyR : ldR;
     numB = serB ? sb : db;
