@@ -348,8 +348,12 @@ rhs_vjp_kernel(int K, const double *__restrict__ vgeo, const double *__restrict_
 }
 
 // (Q_bar, res_bar) <- stage VJP; R_bar written.  Q is the stage's output state.
+// 3D N = 3: three waves per SIMD (what its LDS allows; 168 registers instead of
+// the 172 it takes unconstrained, which leave two).  Other shapes: no constraint.
+template <class S> constexpr int update_vjp_waves() { return (S::DIM == 3 && S::NQ == 23) ? 3 : 1; }
+
 template <class S, int TPW>
-__global__ void __launch_bounds__(WAVE * 4)
+__global__ void __launch_bounds__(WAVE * 4) __attribute__((amdgpu_waves_per_eu(update_vjp_waves<S>())))
 update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ VqPq,
                   const double *__restrict__ VfPq, double fa, double fb, double fdt,
                   const double *__restrict__ Q, double *__restrict__ Qb,
@@ -380,7 +384,7 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
   // A x (row i) and A^T x (column i) with A = VqPq
   auto mv = [&](double (*in)[NQ], double (&r)[NFLD]) {
     for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
-#pragma unroll 2
+#pragma unroll 8
     for (int j = 0; j < NQ; ++j) {
       const double P = sP[i + j * NQ];
       for (int f = 0; f < NFLD; ++f) r[f] += P * in[f][j];
@@ -388,7 +392,7 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
   };
   auto mvT = [&](double (*in)[NQ], double (&r)[NFLD]) {
     for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
-#pragma unroll 2
+#pragma unroll 8
     for (int j = 0; j < NQ; ++j) {
       const double P = sP[j + i * NQ];
       for (int f = 0; f < NFLD; ++f) r[f] += P * in[f][j];
@@ -417,11 +421,12 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
     for (int f = 0; f < NFLD; ++f) bufA[e][f][i] = r[f];  // v~
   }
   __syncthreads();
-  // cotangents of v~ through u(VqPq v~) and u(VfPq v~)
+  // cotangents of v~ through u(v~) and u(VfPq v~): the stage writes rhsQ = u(v~)
+  // (v~ already lies in the range of the projection VqPq)
   double gq[NFLD], gfl[MAXF][NFLD];
   if (valid) {
     double V[NFLD], Ub[NFLD];
-    mv(bufA[e], V);
+    for (int f = 0; f < NFLD; ++f) V[f] = bufA[e][f][i];
     for (int f = 0; f < NFLD; ++f) Ub[f] = uqb[base + i + f * NQ];
     conservative_vars_vjp<DIM>(V, gamma, lgm1, Ub, gq);
     for (int m = 0; m < MAXF; ++m) {
@@ -446,10 +451,10 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
     }
   }
   __syncthreads();
-  // v~_bar = VqPq^T g_q + VfPq^T g_f ; then through VqPq diag(1/J) VqPq
+  // v~_bar = g_q + VfPq^T g_f ; then through VqPq diag(1/J) VqPq
   if (valid) {
     double r[NFLD];
-    mvT(bufB[e], r);
+    for (int f = 0; f < NFLD; ++f) r[f] = bufB[e][f][i];
     for (int fn = 0; fn < NFN; ++fn) {
       const double P = sPf[fn + i * NFN];
       for (int f = 0; f < NFLD; ++f) r[f] += P * bufF[e][f][fn];
