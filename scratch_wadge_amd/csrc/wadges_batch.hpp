@@ -34,7 +34,7 @@ typedef __attribute__((address_space(4))) const BatchGroup ConstGroup;
 template <class... Ss> struct ShapeList {};
 using Family2 = ShapeList<Shape<2, 3, 2>, Shape<2, 6, 3>, Shape<2, 12, 4>, Shape<2, 16, 5>,
                           Shape<2, 25, 6>>;
-using Family3 = ShapeList<Shape<3, 4, 3>, Shape<3, 23, 12>, Shape<3, 44, 16>>;
+using Family3 = ShapeList<Shape<3, 4, 3>, Shape<3, 11, 6>, Shape<3, 23, 12>, Shape<3, 44, 16>>;
 
 template <class S>
 __device__ __forceinline__ void batch_item(const ConstArgs *A, double fa, double fb,

@@ -14,8 +14,11 @@ the reference's:
 The polynomial space is spanned with the orthonormal Dubiner basis directly
 (the reference goes through a nodal basis first); every operator above maps
 quadrature values to quadrature values, so the choice of basis only changes
-round-off.  Quadrature rules are this package's own symmetric rules with the
-reference's point counts (see quadrature_gen.py).
+round-off.  Quadrature rules are this package's own rules with the reference's
+point counts (see quadrature_gen.py): its triangle rules of degree 2-8 are the
+reference's rules, its tetrahedral rules (and the degree-10 triangle rule) are
+other rules of the same size and degree.  use_node_data() takes the rules from a
+reference node_data.h instead.
 """
 
 from __future__ import annotations
@@ -31,6 +34,10 @@ from . import refelem
 
 _RULES_PATH = os.path.join(os.path.dirname(__file__), "quadrature_rules.json")
 
+# Rules read from a reference node_data.h (use_node_data); they take precedence
+# over the stored ones for the degrees they cover.
+_OVERRIDE: dict = {}
+
 
 @functools.lru_cache(maxsize=None)
 def _rules():
@@ -38,16 +45,66 @@ def _rules():
         return json.load(f)
 
 
+def parse_node_data(path):
+    """Quadrature rules tabulated in a reference node_data.h (the tables that
+    tet_cubature / tri_cubature load, operators.cpp:1870-2100): for each degree M
+    present, 'tet{M}' from p_rq_N{M}, p_sq_N{M}, p_tq_N{M}, p_wq_N{M} and 'tri{M}'
+    from p_rfq_N{M}, p_sfq_N{M}, p_wfq_N{M}, on the biunit reference elements
+    (the same as this package's).  Returns {key: {"points": [[..]], "weights": [..]}}."""
+    import re
+    with open(path) as f:
+        src = f.read()
+    arrays = {}
+    for m in re.finditer(r"double\s+(p_\w+?)\s*\[\s*\d+\s*\]\s*=\s*\{([^}]*)\}", src):
+        arrays[m.group(1)] = [float(x) for x in m.group(2).split(",") if x.strip()]
+    rules = {}
+    for name in arrays:
+        m = re.fullmatch(r"p_wq_N(\d+)", name)
+        if m and all(f"p_{c}q_N{m.group(1)}" in arrays for c in "rst"):
+            M = m.group(1)
+            pts = np.stack([arrays[f"p_{c}q_N{M}"] for c in "rst"], 1)
+            rules[f"tet{M}"] = {"points": pts.tolist(), "weights": arrays[name]}
+        m = re.fullmatch(r"p_wfq_N(\d+)", name)
+        if m and all(f"p_{c}fq_N{m.group(1)}" in arrays for c in "rs"):
+            M = m.group(1)
+            pts = np.stack([arrays[f"p_{c}fq_N{M}"] for c in "rs"], 1)
+            rules[f"tri{M}"] = {"points": pts.tolist(), "weights": arrays[name]}
+    return rules
+
+
+def use_node_data(path=None):
+    """Build operators from the quadrature rules of the reference's own
+    node_data.h at `path` (a user's copy of the reference; nothing of it ships
+    with this package), so that the operators equal the reference's up to
+    round-off.  path=None goes back to this package's stored rules.  The
+    environment variable WADGES_NODE_DATA does the same at import.  Returns the
+    rule keys now taken from the file."""
+    _OVERRIDE.clear()
+    if path:
+        _OVERRIDE.update(parse_node_data(path))
+    if "build_ref_ops" in globals():
+        build_ref_ops.cache_clear()   # operators built from the previous rules
+    return sorted(_OVERRIDE)
+
+
+def _rule(key):
+    return _OVERRIDE.get(key) or _rules()[key]
+
+
 def tri_rule(degree):
-    r = _rules()[f"tri{degree}"]
+    r = _rule(f"tri{degree}")
     x = np.array(r["points"])
     return x[:, 0], x[:, 1], np.array(r["weights"])
 
 
 def tet_rule(degree):
-    r = _rules()[f"tet{degree}"]
+    r = _rule(f"tet{degree}")
     x = np.array(r["points"])
     return x[:, 0], x[:, 1], x[:, 2], np.array(r["weights"])
+
+
+if os.environ.get("WADGES_NODE_DATA"):
+    use_node_data(os.environ["WADGES_NODE_DATA"])
 
 
 def available_orders(dim):

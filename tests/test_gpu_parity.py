@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 
-CASES = [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 3), (3, 4)]
+CASES = [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2), (3, 3), (3, 4)]
 
 
 def _solver(prob):

@@ -53,7 +53,7 @@ def test_forward_kernels_do_not_spill(remarks):
         if any(k in name for k in NO_SCRATCH):
             seen += 1
             assert r.get("ScratchSize", 0) == 0, (name[:90], r)
-    assert seen >= 2 * 8 + 2   # vol + stage for 8 shapes, flat ensemble for 2 dims
+    assert seen >= 2 * 9 + 2   # vol + stage for 9 shapes, flat ensemble for 2 dims
 
 
 def test_3d_n3_stage_keeps_two_waves(remarks):

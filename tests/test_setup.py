@@ -26,7 +26,7 @@ def simplex_monomial(dim, exps):
     return tot * 2.0 ** dim
 
 
-@pytest.mark.parametrize("key", ["tri2", "tri4", "tri6", "tri8", "tri10", "tet2", "tet6", "tet8"])
+@pytest.mark.parametrize("key", ["tri2", "tri4", "tri6", "tri8", "tri10", "tet2", "tet4", "tet6", "tet8"])
 def test_rule_exactness(key):
     rules = O._rules()
     r = rules[key]
@@ -45,7 +45,8 @@ def test_rule_exactness(key):
             continue
         q = np.sum(w * np.prod(x ** np.array(exps), 1))
         assert q == pytest.approx(simplex_monomial(dim, exps), abs=1e-13), exps
-    expected = {"tri2": 3, "tri4": 6, "tri6": 12, "tri8": 16, "tri10": 25, "tet2": 4, "tet6": 23, "tet8": 44}
+    expected = {"tri2": 3, "tri4": 6, "tri6": 12, "tri8": 16, "tri10": 25, "tet2": 4, "tet4": 11,
+                "tet6": 23, "tet8": 44}
     assert len(w) == expected[key]
 
 
@@ -61,7 +62,7 @@ def test_triangle_rules_fully_symmetric(deg):
         assert key(lam[:, perm], w) == base
 
 
-@pytest.mark.parametrize("dim,N", [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 3), (3, 4)])
+@pytest.mark.parametrize("dim,N", [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2), (3, 3), (3, 4)])
 def test_hybridized_operators(dim, N):
     ops = O.build_ref_ops(dim, N)
     Np, Nq = ops.Np, ops.Nq
