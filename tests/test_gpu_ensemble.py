@@ -25,7 +25,8 @@ def mixed_2d():
 
 def mixed_3d():
     return [P.tgv3d_problem(n=3, N=3, curved=True), P.tgv3d_problem(n=3, N=4, curved=False),
-            P.tgv3d_problem(n=3, N=1, curved=True), P.tgv3d_problem(n=4, N=3, curved=False)]
+            P.tgv3d_problem(n=3, N=1, curved=True), P.tgv3d_problem(n=4, N=3, curved=False),
+            P.tgv3d_problem(n=3, N=2, curved=True)]
 
 
 def compare(ens, ref, label):
