@@ -42,8 +42,10 @@ def parse():
                     help="also measure the ensemble and adjoint workloads (default)")
     ap.add_argument("--no-extras", dest="extras", action="store_false")
     ap.add_argument("--ensemble-steps", type=int, default=100)
-    ap.add_argument("--leg", choices=("all", "ensemble", "adjoint"), default="all",
-                    help="ensemble / adjoint: run only that leg and print its JSON object")
+    ap.add_argument("--leg", choices=("all", "ensemble", "adjoint", "adjoint3d"), default="all",
+                    help="ensemble / adjoint: run only that leg and print its JSON object; "
+                         "adjoint3d: the adjoint sweep on the 3D curved K = 196,608, N = 3 mesh "
+                         "(not part of the default run)")
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="instead of the benchmark, copy a GIB-sized device buffer 5 times "
@@ -166,6 +168,35 @@ def ensemble_bench(steps, seed=0):
             "bitwise_equal_to_per_problem": bool(equal), "finite": bool(finite)}
 
 
+def adjoint3d_bench(steps=8, ckpt=4, n=32):
+    """The adjoint sweep at the headline scale: 3D curved Taylor-Green, n^3 x 6
+    tets (K = 196,608 at n = 32), N = 3, `steps` LSRK45 steps with checkpoints
+    every `ckpt`, cotangent of all ones at the final state; sweep time against
+    the plain forward (both replayed from HIP graphs)."""
+    import torch
+    from scratch_wadge_amd import problem as P
+    from scratch_wadge_amd.adjoint import AdjointSolver
+    from scratch_wadge_amd.device import DeviceSolver
+    prob = P.tgv3d_problem(n=n, N=3, curved=True)
+    dev = DeviceSolver(prob)
+    dev.run(2, fused=True, graph=True)              # warm-up, graph capture
+    t_plain = _sync_time(lambda: dev.run(steps, fused=True, graph=True))
+    adj = AdjointSolver(dev, checkpoint_every=ckpt)
+    Q0 = torch.as_tensor(prob.Q, device=dev.device)
+    adj.forward(Q0, steps)                          # warm-up, graph captures
+    adj.backward(torch.zeros_like(Q0))
+    out = {}
+    t_fwd = _sync_time(lambda: out.setdefault("QT", adj.forward(Q0, steps)))
+    t_bwd = _sync_time(lambda: out.setdefault("g", adj.backward(torch.ones_like(Q0))))
+    g = out["g"]
+    return {"workload": f"3D curved TGV {n}^3 x 6 (K={prob.K}), N=3, {steps} steps, "
+                        f"checkpoints every {ckpt}",
+            "forward_ms": round(t_plain * 1e3, 2), "forward_with_checkpoints_ms": round(t_fwd * 1e3, 2),
+            "adjoint_sweep_ms": round(t_bwd * 1e3, 2),
+            "adjoint_over_forward": round(t_bwd / t_plain, 3),
+            "grad_finite": bool(torch.isfinite(g).all()), "grad_norm": float(torch.linalg.norm(g))}
+
+
 def adjoint_bench(steps, ckpt=40):
     """2D vortex, 32x32x2 = 2048 triangles, N=3: density misfit after `steps`
     LSRK45 steps against a target from Gaussian-bump-perturbed initial density;
@@ -212,6 +243,9 @@ def main():
         return
     if args.leg == "adjoint":
         print(json.dumps({"adjoint": adjoint_bench(args.adjoint_steps)}), flush=True)
+        return
+    if args.leg == "adjoint3d":
+        print(json.dumps({"adjoint3d": adjoint3d_bench()}), flush=True)
         return
     import numpy as np
     import torch
