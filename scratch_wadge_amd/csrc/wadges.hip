@@ -1285,8 +1285,8 @@ template <class S> int update_vjp_impl(const wadges_plan *p, int K, const double
                                        hipStream_t st) {
   if (K == 0) return WADGES_OK;
   constexpr int TPW = WAVE / S::NQ;
-  constexpr int EPB = 4 * TPW;
-  update_vjp_kernel<S, TPW><<<(K + EPB - 1) / EPB, WAVE * 4, 0, st>>>(
+  constexpr int EPB = uvjp_wg_waves<S>() * TPW;
+  update_vjp_kernel<S, TPW><<<(K + EPB - 1) / EPB, WAVE * uvjp_wg_waves<S>(), 0, st>>>(
       K, Jq, VqPq, VfPq, fa, fb, fdt, Q, Qb, resb, uqb, ufb, Rb, p->gamma, std::log(p->gamma - 1.0),
       nullptr, nullptr);
   return hip_check(hipGetLastError());
@@ -1311,8 +1311,8 @@ template <class S> int stage_vjp_impl(const wadges_plan *p, int K, const double 
   const double g = p->gamma;
   {
     constexpr int TPW = WAVE / S::NQ;
-    constexpr int EPB = 4 * TPW;
-    update_vjp_kernel<S, TPW><<<(K + EPB - 1) / EPB, WAVE * 4, 0, st>>>(
+    constexpr int EPB = uvjp_wg_waves<S>() * TPW;
+    update_vjp_kernel<S, TPW><<<(K + EPB - 1) / EPB, WAVE * uvjp_wg_waves<S>(), 0, st>>>(
         K, Jq, VqPq, VfPq, fa, fb, fdt, Q_out, Qb, resb, uqb_out, ufb_out, Rb, g,
         std::log(g - 1.0), VqLq, sbar);
     if ((rc = hip_check(hipGetLastError()))) return rc;

@@ -348,12 +348,16 @@ rhs_vjp_kernel(int K, const double *__restrict__ vgeo, const double *__restrict_
 }
 
 // (Q_bar, res_bar) <- stage VJP; R_bar written.  Q is the stage's output state.
-// 3D N = 3: three waves per SIMD (what its LDS allows; 168 registers instead of
-// the 172 it takes unconstrained, which leave two).  Other shapes: no constraint.
+// Workgroup shape of the update VJP.  2D: one wave (its barriers cost nothing),
+// the small operators read through L1.  3D: four waves sharing the operators
+// staged in LDS (the transposed products read them along rows, which through
+// L1 touches a cache line per lane).  3D N = 3 declares three waves per SIMD
+// (168 registers instead of the 172 it takes unconstrained, which leave two).
+template <class S> constexpr int uvjp_wg_waves() { return S::DIM == 2 ? 1 : 4; }
 template <class S> constexpr int update_vjp_waves() { return (S::DIM == 3 && S::NQ == 23) ? 3 : 1; }
 
 template <class S, int TPW>
-__global__ void __launch_bounds__(WAVE * 4) __attribute__((amdgpu_waves_per_eu(update_vjp_waves<S>())))
+__global__ void __launch_bounds__(WAVE * uvjp_wg_waves<S>()) __attribute__((amdgpu_waves_per_eu(update_vjp_waves<S>())))
 update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ VqPq,
                   const double *__restrict__ VfPq, double fa, double fb, double fdt,
                   const double *__restrict__ Q, double *__restrict__ Qb,
@@ -361,15 +365,25 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
                   const double *__restrict__ ufb, double *__restrict__ Rb, double gamma,
                   double lgm1, const double *__restrict__ VqLq, double *__restrict__ sbar) {
   constexpr int DIM = S::DIM, NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD;
-  constexpr int EPB = 4 * TPW;
+  constexpr int WPB = uvjp_wg_waves<S>();
+  constexpr bool OPS_LDS = WPB > 1;
+  constexpr int EPB = WPB * TPW;
   constexpr int MAXF = (NFN + NQ - 1) / NQ;
-  __shared__ double sP[NQ * NQ];
-  __shared__ double sPf[NFN * NQ];
+  __shared__ double sP[OPS_LDS ? NQ * NQ : 1];
+  __shared__ double sPf[OPS_LDS ? NFN * NQ : 1];
   __shared__ double bufA[EPB][NFLD][NQ];
   __shared__ double bufB[EPB][NFLD][NQ];
   __shared__ double bufF[EPB][NFLD][NFN];
-  for (int idx = threadIdx.x; idx < NQ * NQ; idx += blockDim.x) sP[idx] = VqPq[idx];
-  for (int idx = threadIdx.x; idx < NFN * NQ; idx += blockDim.x) sPf[idx] = VfPq[idx];
+  if constexpr (OPS_LDS) {
+    for (int idx = threadIdx.x; idx < NQ * NQ; idx += blockDim.x) sP[idx] = VqPq[idx];
+    for (int idx = threadIdx.x; idx < NFN * NQ; idx += blockDim.x) sPf[idx] = VfPq[idx];
+  }
+  auto Pv = [&](int idx) {      // VqPq, column-major
+    if constexpr (OPS_LDS) return sP[idx]; else return ld_off(VqPq, (unsigned)idx);
+  };
+  auto Pf = [&](int idx) {      // VfPq, column-major
+    if constexpr (OPS_LDS) return sPf[idx]; else return ld_off(VfPq, (unsigned)idx);
+  };
   const int wave = threadIdx.x / WAVE;
   const int wl = threadIdx.x % WAVE;
   const int tw = wl / NQ;
@@ -386,7 +400,7 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
     for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
 #pragma unroll 8
     for (int j = 0; j < NQ; ++j) {
-      const double P = sP[i + j * NQ];
+      const double P = Pv(i + j * NQ);
       for (int f = 0; f < NFLD; ++f) r[f] += P * in[f][j];
     }
   };
@@ -394,7 +408,7 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
     for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
 #pragma unroll 8
     for (int j = 0; j < NQ; ++j) {
-      const double P = sP[j + i * NQ];
+      const double P = Pv(j + i * NQ);
       for (int f = 0; f < NFLD; ++f) r[f] += P * in[f][j];
     }
   };
@@ -434,7 +448,7 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
       if (fn >= NFN) continue;
       for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
       for (int j = 0; j < NQ; ++j) {
-        const double P = sPf[fn + j * NFN];
+        const double P = Pf(fn + j * NFN);
         for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[e][f][j];
       }
       for (int f = 0; f < NFLD; ++f) Ub[f] = ufb[basef + fn + f * NFN];
@@ -456,7 +470,7 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
     double r[NFLD];
     for (int f = 0; f < NFLD; ++f) r[f] = bufB[e][f][i];
     for (int fn = 0; fn < NFN; ++fn) {
-      const double P = sPf[fn + i * NFN];
+      const double P = Pf(fn + i * NFN);
       for (int f = 0; f < NFLD; ++f) r[f] += P * bufF[e][f][fn];
     }
     for (int f = 0; f < NFLD; ++f) bufA[e][f][i] = r[f];
