@@ -165,3 +165,33 @@ def test_vortex_ensemble_generator():
         assert p.K == 2 * n * n and 64 <= p.K <= 2 * 64 * 64
         assert p.N in (2, 3, 4) and p.dim == 2 and p.dt > 0
     assert np.array_equal(a[0].Q, b[0].Q)
+
+
+def test_node_data_parser(tmp_path):
+    """operators.parse_node_data / use_node_data on a header in the reference's
+    table format (C arrays p_rq_N{M}, ... p_wfq_N{M}), written here from this
+    package's own degree-2 rules."""
+    tri = O._rules()["tri2"]
+    tet = O._rules()["tet2"]
+    x, w = np.array(tri["points"]), np.array(tri["weights"])
+    X, W = np.array(tet["points"]), np.array(tet["weights"])
+
+    def arr(name, v):
+        return f"double {name}[{len(v)}] = {{" + " , ".join(f"{a:.17g}" for a in v) + "};\n"
+
+    src = "int p_Nq_N2 = 4;\n" + arr("p_rq_N2", X[:, 0]) + arr("p_sq_N2", X[:, 1]) + \
+        arr("p_tq_N2", X[:, 2]) + arr("p_wq_N2", W) + arr("p_rfq_N2", x[:, 0]) + \
+        arr("p_sfq_N2", x[:, 1]) + arr("p_wfq_N2", w) + arr("p_r_N2", X[:, 0])
+    path = tmp_path / "node_data.h"
+    path.write_text(src)
+    rules = O.parse_node_data(str(path))
+    assert sorted(rules) == ["tet2", "tri2"]
+    np.testing.assert_array_equal(np.array(rules["tet2"]["points"]), X)
+    np.testing.assert_array_equal(np.array(rules["tri2"]["weights"]), w)
+    try:
+        assert O.use_node_data(str(path)) == ["tet2", "tri2"]
+        ops = O.build_ref_ops(3, 1)
+        np.testing.assert_array_equal(ops.wq, W)
+    finally:
+        O.use_node_data(None)
+    assert O.build_ref_ops(3, 1).Nq == 4
