@@ -590,15 +590,16 @@ surf_kernel(int K, const double *__restrict__ fgeo, const uint32_t *__restrict__
     double nJ[DIM];
     for (int d = 0; d < DIM; ++d) nJ[d] = g[fn + d * NFN];
     const double sJ = g[fn + DIM * NFN];
-    const Node<DIM> a = make_node<DIM>(UM, gm1);
-    const Node<DIM> b = make_node<DIM>(UP, gm1);
+    // 3D: the logarithms are taken only if a lane of the wave needs the log branch
+    const Node<DIM> a = make_node<DIM, DIM == 2>(UM, gm1);
+    const Node<DIM> b = make_node<DIM, DIM == 2>(UP, gm1);
     // wave speed |u| + sqrt(gamma p / rho)  (okl/Euler3D.okl:3316-3321); 2 hq = |u|^2 exactly
     const double pM = gm1 * (UM[NFLD - 1] - 0.5 * UM[0] * (2.0 * a.hq));
     const double pP = gm1 * (UP[NFLD - 1] - 0.5 * UP[0] * (2.0 * b.hq));
     const double lamM = sqrt(2.0 * a.hq) + sqrt(gamma * pM / UM[0]);
     const double lamP = sqrt(2.0 * b.hq) + sqrt(gamma * pP / UP[0]);
     const double Lfc = ltau * fmax(lamM, lamP) * sJ;
-    const PairFlux<DIM> P = pair_flux<DIM>(a, b, inv2gm1);
+    const PairFlux<DIM> P = pair_flux<DIM, DIM == 3>(a, b, inv2gm1);
     double Fn[NFLD];
     flux_dot<DIM>(P, nJ, Fn);
     const double *rf = rhsQf + (size_t)k * NFN * NFLD;
@@ -838,14 +839,15 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
     double nJ[DIM];
     for (int d = 0; d < DIM; ++d) nJ[d] = g[fn + d * NFN];
     const double sJ = g[fn + DIM * NFN];
-    const Node<DIM> a = make_node<DIM>(UM, gm1);
-    const Node<DIM> b = make_node<DIM>(UP, gm1);
+    // 3D: the logarithms are taken only if a lane of the wave needs the log branch
+    const Node<DIM> a = make_node<DIM, DIM == 2>(UM, gm1);
+    const Node<DIM> b = make_node<DIM, DIM == 2>(UP, gm1);
     const double pM = gm1 * (UM[NFLD - 1] - 0.5 * UM[0] * (2.0 * a.hq));
     const double pP = gm1 * (UP[NFLD - 1] - 0.5 * UP[0] * (2.0 * b.hq));
     const double lamM = sqrt(2.0 * a.hq) + sqrt(gamma * pM / UM[0]);
     const double lamP = sqrt(2.0 * b.hq) + sqrt(gamma * pP / UP[0]);
     const double Lfc = ltau * fmax(lamM, lamP) * sJ;
-    const PairFlux<DIM> P = pair_flux<DIM>(a, b, inv2gm1);
+    const PairFlux<DIM> P = pair_flux<DIM, DIM == 3>(a, b, inv2gm1);
     double Fn[NFLD];
     flux_dot<DIM>(P, nJ, Fn);
     for (int f = 0; f < NFLD; ++f)

@@ -52,7 +52,10 @@ __device__ __forceinline__ double rcp(double x) {
 
 // Conservative state -> node record.  3D multiplies by 1/rho
 // (okl/Euler3D.okl:2252-2256, TESTING); 2D divides (okl/Euler2D.okl:1127-1129).
-template <int DIM>
+// LOGS = false leaves ln rho and ln beta out, for a node that serves a single
+// pair: pair_flux<DIM, true> then takes the logarithms only where a log-mean
+// log branch needs them.
+template <int DIM, bool LOGS = true>
 __device__ __forceinline__ Node<DIM> make_node(const double (&U)[DIM + 2], double gm1) {
   Node<DIM> n;
   n.rho = U[0];
@@ -71,8 +74,8 @@ __device__ __forceinline__ Node<DIM> make_node(const double (&U)[DIM + 2], doubl
   else
     n.beta = 0.5 * U[0] / p;
   for (int d = 0; d < DIM; ++d) n.hv[d] = 0.5 * v[d];
-  n.lrho = log(n.rho);
-  n.lbeta = log(n.beta);
+  n.lrho = LOGS ? log(n.rho) : 0.0;
+  n.lbeta = LOGS ? log(n.beta) : 0.0;
   n.hq = 0.5 * uu;
   return n;
 }
@@ -112,7 +115,7 @@ __device__ __forceinline__ bool wave_all(bool p) {
 // allows), where the selection is skipped.  Both ways give the same bits.  (In
 // 2D the kernels aim at four or more waves per SIMD, whose register budget
 // cannot hold both branches' values across the test: it spilled.)
-template <int DIM>
+template <int DIM, bool LAZY_LOGS = false>
 __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Node<DIM> &b,
                                                    double inv2gm1) {
   constexpr double TH = (DIM == 3) ? 1.0e-4 : 1.0e-3;
@@ -125,12 +128,12 @@ __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Nod
   const double ur = fr * fr, ub = fb * fb;
   const double polyR = logmean_series2<DIM>(ur);
   const double polyB = logmean_series2<DIM>(ub);
-  const double ldR = a.lrho - b.lrho;
-  const double ldB = a.lbeta - b.lbeta;
   const bool serR = ur < TH, serB = ub < TH;
   // logmean = num / den:  series (aL+aR) / (2 F(u)),  log (aL-aR) / (ln aL - ln aR)
   double numR = sr, denR = polyR, numB = sb, denB = polyB;
   if (DIM == 2 || !wave_all(serR && serB)) {
+    const double ldR = LAZY_LOGS ? log(a.rho) - log(b.rho) : a.lrho - b.lrho;
+    const double ldB = LAZY_LOGS ? log(a.beta) - log(b.beta) : a.lbeta - b.lbeta;
     numR = serR ? sr : dr;
     denR = serR ? polyR : ldR;
     numB = serB ? sb : db;
