@@ -148,7 +148,7 @@ def test_trajectory_gradient_fd(dim, N, steps, ckpt):
 
 # ----------------------------------------------------------------- reference gradient
 
-@pytest.mark.parametrize("dim,N", [(2, 2), (2, 3), (3, 3), (3, 4)])
+@pytest.mark.parametrize("dim,N", [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2), (3, 3), (3, 4)])
 def test_rhs_vjp_matches_reference_gradient(dim, N):
     from scratch_wadge_amd.device import _ptr
     prob = small_problem(dim, N)
@@ -177,7 +177,7 @@ def test_rhs_vjp_matches_reference_gradient(dim, N):
     assert e1 < 1e-11 and e2 < 1e-11
 
 
-@pytest.mark.parametrize("dim,N", [(2, 3), (3, 3)])
+@pytest.mark.parametrize("dim,N", [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2), (3, 3), (3, 4)])
 def test_update_vjp_matches_reference_gradient(dim, N):
     from scratch_wadge_amd.device import _ptr
     from scratch_wadge_amd.problem import RK4A, RK4B
