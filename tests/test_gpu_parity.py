@@ -1,6 +1,6 @@
 """GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
 
-Same inputs on both sides; the bar is the north-star RHS tolerance: relative
+Same inputs on both sides; the bar is the path's RHS tolerance: relative
 L2 difference <= 1e-12 for the volume, surface and update outputs, stage by
 stage.
 """
