@@ -105,9 +105,13 @@ batch_stage_persistent(const BatchGroup *__restrict__ groups, const int2 *__rest
 }
 
 // One workgroup per item of one tile: no loop, the register use of the
-// single-problem stage kernel.
+// single-problem stage kernels (the largest over the family).  2D: four waves
+// per SIMD declared (the 2D family takes 132 registers unconstrained, which
+// leaves three); 3D: no constraint.
+template <int DIM> constexpr int flat_waves() { return DIM == 2 ? 4 : 1; }
+
 template <int DIM, class Fam>
-__global__ void __launch_bounds__(WAVE)
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(flat_waves<DIM>())))
 batch_stage_flat(const BatchGroup *__restrict__ groups, const int2 *__restrict__ items,
                  double fa, double fb, int cur) {
   extern __shared__ double smem[];

@@ -1,9 +1,11 @@
 """Register budget of the forward kernels, from the compiler's resource remarks
 (-Rpass-analysis=kernel-resource-usage) on a device-only compile with the
-library's own flags.  The volume, fused stage and one-tile ensemble kernels of
-every shape must not spill to scratch (a spill there costs more than any
-register-hungry schedule gains), and the 3D N = 3 fused stage must keep the two
-waves per SIMD that its LDS allows."""
+library's own flags.  The volume and fused stage kernels of every shape must not
+spill to scratch (a spill there costs more than any register-hungry schedule
+gains); the one-tile ensemble kernels may spill at most 32 bytes per lane (the
+2D one trades 20 bytes for a fourth wave per SIMD, measured 6 % faster on the
+64-problem ensemble); the 3D N = 3 fused stage must keep the two waves per SIMD
+that its LDS allows."""
 
 import os
 import re
@@ -14,7 +16,8 @@ import pytest
 
 from scratch_wadge_amd import build as B
 
-NO_SCRATCH = ("vol_kernel", "stage_kernel", "batch_stage_flat")
+NO_SCRATCH = ("vol_kernel", "stage_kernel")
+SMALL_SCRATCH = ("batch_stage_flat",)
 
 
 @pytest.fixture(scope="module")
@@ -48,12 +51,15 @@ def _shape(name):
 
 
 def test_forward_kernels_do_not_spill(remarks):
-    seen = 0
+    seen = flat = 0
     for name, r in remarks.items():
         if any(k in name for k in NO_SCRATCH):
             seen += 1
             assert r.get("ScratchSize", 0) == 0, (name[:90], r)
-    assert seen >= 2 * 9 + 2   # vol + stage for 9 shapes, flat ensemble for 2 dims
+        if any(k in name for k in SMALL_SCRATCH):
+            flat += 1
+            assert r.get("ScratchSize", 0) <= 32, (name[:90], r)
+    assert seen >= 2 * 9 and flat == 2   # vol + stage for 9 shapes, flat ensemble for 2 dims
 
 
 def test_3d_n3_stage_keeps_two_waves(remarks):
