@@ -72,8 +72,9 @@ int wadges_plan_destroy(wadges_plan *plan);
 /* Forgets the prepared operator tables (see above); the next call prepares them. */
 int wadges_plan_invalidate(wadges_plan *plan);
 
-/* Volume kernel: role of euler_vol_3d_curved (okl/Euler3D.okl:2123-2523) and
- * euler_vol_2d_curved (okl/Euler2D.okl:773-1353).  Reads the entropy-projected
+/* Volume kernel: role of euler_vol_3d_curved (okl/Euler3D.okl:2123-2523),
+ * euler_vol_2d_curved (okl/Euler2D.okl:773-1353) and, on affine meshes, of
+ * euler_vol_3d (okl/Euler3Daffine.okl:152, same arguments).  Reads the entropy-projected
  * volume state from rhsQ and the face state from Qf; overwrites rhsQ with the
  * flux-differencing volume term and writes the volume-to-face coupling term to
  * rhsQf.  Q and Drstq are accepted for argument compatibility and not read (the
@@ -84,7 +85,8 @@ int wadges_vol(const wadges_plan *plan, int K, const double *vgeo, const double 
                const double *VfPq, const double *Q, const double *Qf, double *rhsQ,
                double *rhsQf, void *stream);
 
-/* Surface kernel: role of euler_surf_3d (okl/Euler3D.okl:3242-3473) and
+/* Surface kernel: role of euler_surf_3d (okl/Euler3D.okl:3242-3473; the
+ * affine euler_surf_3d, okl/Euler3Daffine.okl:421, has the same arguments) and
  * euler_surf_2d (okl/Euler2D.okl:1357-1495).  Entropy-conservative flux plus
  * Lax-Friedrichs penalty at every face point, plus rhsQf, lifted by VqLq and
  * added to rhsQ. */
@@ -92,8 +94,9 @@ int wadges_surf(const wadges_plan *plan, int K, const double *fgeo, const uint32
                 const double *VqLq, const double *Qf, const double *rhsQf, double *rhsQ,
                 void *stream);
 
-/* Update kernel: role of euler_update_3d_curved (okl/Euler3D.okl:2527-2884) and
- * euler_update_2d_curved (okl/Euler2D.okl:1729-2282).  Weight-adjusted inverse
+/* Update kernel: role of euler_update_3d_curved (okl/Euler3D.okl:2527-2884),
+ * euler_update_2d_curved (okl/Euler2D.okl:1729-2282) and, on affine meshes, of
+ * euler_update_3d (okl/Euler3Daffine.okl:555, same arguments).  Weight-adjusted inverse
  * mass -2 VqPq diag(1/J) VqPq, low-storage RK update of resQ and Q, then the
  * weight-adjusted entropy projection: Qf = u(VfPq v~), rhsQ = u(VqPq v~). */
 int wadges_update(const wadges_plan *plan, int K, const double *Jq, const double *VqPq,
