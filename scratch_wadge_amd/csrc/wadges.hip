@@ -624,14 +624,27 @@ surf_kernel(int K, const double *__restrict__ fgeo, const uint32_t *__restrict__
 
 // ------------------------------------------------------------------ update kernel
 
-template <class S, int TPW>
+// Launch shape of update_kernel: lane l of an element's LPE lanes holds rows
+// l + r * LPE (r < R) of the matrix-vector products, so each value broadcast
+// from LDS feeds R rows.  At 3D N = 3 two rows per lane put 5 elements of 12
+// lanes in a wave (60 lanes busy) instead of 2 of 23 (46).  Each row sums over
+// j in ascending order whatever R is, so R changes no result.
+template <class S> struct UpdShape {
+  static constexpr int R = (S::DIM == 3 && S::NQ == 23) ? 2 : 1;
+  static constexpr int LPE = (S::NQ + R - 1) / R;
+  static constexpr int TPW = WAVE / LPE;
+  static constexpr int EPB = 4 * TPW;   // four waves per workgroup
+};
+
+template <class S>
 __global__ void __launch_bounds__(WAVE * 4)
 update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ VqPq,
               const double *__restrict__ VfPq, double fa, double fb, double fdt,
               double *__restrict__ rhsQ, double *__restrict__ resQ, double *__restrict__ Q,
               double *__restrict__ Qf, double gamma, double lgm1) {
   constexpr int DIM = S::DIM, NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD;
-  constexpr int EPB = 4 * TPW;
+  constexpr int R = UpdShape<S>::R, LPE = UpdShape<S>::LPE, TPW = UpdShape<S>::TPW;
+  constexpr int EPB = UpdShape<S>::EPB;
   __shared__ double sP[NQ * NQ];
   __shared__ double sPf[NFN * NQ];
   __shared__ double bufA[EPB][NFLD][NQ];
@@ -641,82 +654,98 @@ update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ V
 
   const int wave = threadIdx.x / WAVE;
   const int wl = threadIdx.x % WAVE;
-  const int tw = wl / NQ;
-  const int i = wl - tw * NQ;
+  const int tw = wl / LPE;
+  const int l = wl - tw * LPE;
   const bool active = tw < TPW;
   const int e = wave * TPW + (active ? tw : 0);
   const int k = blockIdx.x * EPB + e;
   const bool valid = active && k < K;
   const size_t base = (size_t)k * NQ * NFLD;
+  // rows of this lane; a row beyond NQ (last lane of an element) is not computed
+  int ir[R];
+  bool rv[R];
+  for (int x = 0; x < R; ++x) {
+    ir[x] = l + x * LPE;
+    rv[x] = valid && ir[x] < NQ;
+    if (!rv[x]) ir[x] = l;
+  }
+  // out[x][f] = sum_j c M[row x, j] src[f][j], M = VqPq, j ascending
+  auto mv = [&](double (*src)[NQ], double c, double (&out)[R][NFLD]) {
+    for (int x = 0; x < R; ++x)
+      for (int f = 0; f < NFLD; ++f) out[x][f] = 0.0;
+#pragma unroll 2
+    for (int j = 0; j < NQ; ++j) {
+      double d[NFLD];
+      for (int f = 0; f < NFLD; ++f) d[f] = src[f][j];
+      for (int x = 0; x < R; ++x) {
+        const double P = c * sP[ir[x] + j * NQ];
+        for (int f = 0; f < NFLD; ++f) out[x][f] += P * d[f];
+      }
+    }
+  };
 
-  double J = 1.0, invJ = 1.0;
-  if (valid) {
-    J = Jq[(size_t)k * NQ + i];
-    invJ = 1.0 / J;
-    for (int f = 0; f < NFLD; ++f) bufA[e][f][i] = rhsQ[base + i + f * NQ];
+  double J[R], invJ[R];
+  for (int x = 0; x < R; ++x) {
+    J[x] = invJ[x] = 1.0;
+    if (rv[x]) {
+      J[x] = Jq[(size_t)k * NQ + ir[x]];
+      invJ[x] = 1.0 / J[x];
+      for (int f = 0; f < NFLD; ++f) bufA[e][f][ir[x]] = rhsQ[base + ir[x] + f * NQ];
+    }
   }
   __syncthreads();
   // r = -2 VqPq rhs ; bufB = r / J
   if (valid) {
-    double r[NFLD];
-    for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
-#pragma unroll 2
-    for (int j = 0; j < NQ; ++j) {
-      const double P = -2.0 * sP[i + j * NQ];
-      for (int f = 0; f < NFLD; ++f) r[f] += P * bufA[e][f][j];
-    }
-    for (int f = 0; f < NFLD; ++f) bufB[e][f][i] = (DIM == 3) ? r[f] * invJ : r[f] / J;
+    double r[R][NFLD];
+    mv(bufA[e], -2.0, r);
+    for (int x = 0; x < R; ++x)
+      if (rv[x])
+        for (int f = 0; f < NFLD; ++f)
+          bufB[e][f][ir[x]] = (DIM == 3) ? r[x][f] * invJ[x] : r[x][f] / J[x];
   }
   __syncthreads();
   // r = VqPq bufB ; RK update ; bufA = v(Q) J
   if (valid) {
-    double r[NFLD];
-    for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
-#pragma unroll 2
-    for (int j = 0; j < NQ; ++j) {
-      const double P = sP[i + j * NQ];
-      for (int f = 0; f < NFLD; ++f) r[f] += P * bufB[e][f][j];
+    double r[R][NFLD];
+    mv(bufB[e], 1.0, r);
+    for (int x = 0; x < R; ++x) {
+      if (!rv[x]) continue;
+      double U[NFLD], V[NFLD];
+      for (int f = 0; f < NFLD; ++f) {
+        const size_t id = base + ir[x] + f * NQ;
+        const double res = fa * resQ[id] + fdt * r[x][f];
+        resQ[id] = res;
+        U[f] = Q[id] + fb * res;
+        Q[id] = U[f];
+      }
+      entropy_vars<DIM>(U, gamma, V);
+      for (int f = 0; f < NFLD; ++f) bufA[e][f][ir[x]] = V[f] * J[x];
     }
-    double U[NFLD], V[NFLD];
-    for (int f = 0; f < NFLD; ++f) {
-      const size_t id = base + i + f * NQ;
-      const double res = fa * resQ[id] + fdt * r[f];
-      resQ[id] = res;
-      U[f] = Q[id] + fb * res;
-      Q[id] = U[f];
-    }
-    entropy_vars<DIM>(U, gamma, V);
-    for (int f = 0; f < NFLD; ++f) bufA[e][f][i] = V[f] * J;
   }
   __syncthreads();
   // bufB = VqPq bufA / J
   if (valid) {
-    double r[NFLD];
-    for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
-#pragma unroll 2
-    for (int j = 0; j < NQ; ++j) {
-      const double P = sP[i + j * NQ];
-      for (int f = 0; f < NFLD; ++f) r[f] += P * bufA[e][f][j];
-    }
-    for (int f = 0; f < NFLD; ++f) bufB[e][f][i] = (DIM == 3) ? r[f] * invJ : r[f] / J;
+    double r[R][NFLD];
+    mv(bufA[e], 1.0, r);
+    for (int x = 0; x < R; ++x)
+      if (rv[x])
+        for (int f = 0; f < NFLD; ++f)
+          bufB[e][f][ir[x]] = (DIM == 3) ? r[x][f] * invJ[x] : r[x][f] / J[x];
   }
   __syncthreads();
   // bufA = VqPq bufB   (entropy variables v~)
   if (valid) {
-    double r[NFLD];
-    for (int f = 0; f < NFLD; ++f) r[f] = 0.0;
-#pragma unroll 2
-    for (int j = 0; j < NQ; ++j) {
-      const double P = sP[i + j * NQ];
-      for (int f = 0; f < NFLD; ++f) r[f] += P * bufB[e][f][j];
-    }
-    for (int f = 0; f < NFLD; ++f) bufA[e][f][i] = r[f];
+    double r[R][NFLD];
+    mv(bufB[e], 1.0, r);
+    for (int x = 0; x < R; ++x)
+      if (rv[x])
+        for (int f = 0; f < NFLD; ++f) bufA[e][f][ir[x]] = r[x][f];
   }
   __syncthreads();
   if (valid) {
     // face points: Qf = u(VfPq v~)
     const size_t basef = (size_t)k * NFN * NFLD;
-    for (int fn = i; fn < NFN; fn += NQ) {
+    for (int fn = l; fn < NFN; fn += LPE) {
       double V[NFLD], U[NFLD];
       for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
 #pragma unroll 2
@@ -729,10 +758,13 @@ update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ V
     }
     // volume points: rhsQ = u(VqPq v~) = u(v~) (VqPq is a projection and v~
     // already lies in its range, round-off apart)
-    double V[NFLD], U[NFLD];
-    for (int f = 0; f < NFLD; ++f) V[f] = bufA[e][f][i];
-    conservative_vars<DIM>(V, gamma, lgm1, U);
-    for (int f = 0; f < NFLD; ++f) rhsQ[base + i + f * NQ] = U[f];
+    for (int x = 0; x < R; ++x) {
+      if (!rv[x]) continue;
+      double V[NFLD], U[NFLD];
+      for (int f = 0; f < NFLD; ++f) V[f] = bufA[e][f][ir[x]];
+      conservative_vars<DIM>(V, gamma, lgm1, U);
+      for (int f = 0; f < NFLD; ++f) rhsQ[base + ir[x] + f * NQ] = U[f];
+    }
   }
 }
 
@@ -1208,11 +1240,10 @@ template <class S> int update_impl(const wadges_plan *p, int K, const double *Jq
                                    double fdt, double *rhsQ, double *resQ, double *Q, double *Qf,
                                    hipStream_t st) {
   if (K == 0) return WADGES_OK;
-  constexpr int TPW = WAVE / S::NQ;
-  constexpr int EPB = 4 * TPW;
+  constexpr int EPB = UpdShape<S>::EPB;
   const int blocks = (K + EPB - 1) / EPB;
-  update_kernel<S, TPW><<<blocks, WAVE * 4, 0, st>>>(K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ,
-                                                     Q, Qf, p->gamma, std::log(p->gamma - 1.0));
+  update_kernel<S><<<blocks, WAVE * 4, 0, st>>>(K, Jq, VqPq, VfPq, fa, fb, fdt, rhsQ, resQ, Q, Qf,
+                                                p->gamma, std::log(p->gamma - 1.0));
   return hip_check(hipGetLastError());
 }
 
