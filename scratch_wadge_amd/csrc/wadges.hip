@@ -627,13 +627,18 @@ surf_kernel(int K, const double *__restrict__ fgeo, const uint32_t *__restrict__
 // Launch shape of update_kernel: lane l of an element's LPE lanes holds rows
 // l + r * LPE (r < R) of the matrix-vector products, so each value broadcast
 // from LDS feeds R rows.  At 3D N = 3 two rows per lane put 5 elements of 12
-// lanes in a wave (60 lanes busy) instead of 2 of 23 (46).  Each row sums over
-// j in ascending order whatever R is, so R changes no result.
+// lanes in a wave (60 lanes busy) instead of 2 of 23 (46); at 3D N = 4 two
+// elements of 22 lanes instead of one of 44.  Each row sums over j in ascending
+// order whatever R is, so R changes no result.  VfPq is staged in LDS where the
+// workgroup's LDS stays within 64 KB; otherwise the face products read it
+// through L1 (consecutive lanes, consecutive face points: contiguous).
 template <class S> struct UpdShape {
-  static constexpr int R = (S::DIM == 3 && S::NQ == 23) ? 2 : 1;
+  static constexpr int R = (S::DIM == 3 && (S::NQ == 23 || S::NQ == 44)) ? 2 : 1;
   static constexpr int LPE = (S::NQ + R - 1) / R;
   static constexpr int TPW = WAVE / LPE;
   static constexpr int EPB = 4 * TPW;   // four waves per workgroup
+  static constexpr bool PF_LDS =
+      8 * (S::NQ * S::NQ + S::NFN * S::NQ + 2 * EPB * S::NFLD * S::NQ) <= 65536;
 };
 
 template <class S>
@@ -645,12 +650,14 @@ update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ V
   constexpr int DIM = S::DIM, NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD;
   constexpr int R = UpdShape<S>::R, LPE = UpdShape<S>::LPE, TPW = UpdShape<S>::TPW;
   constexpr int EPB = UpdShape<S>::EPB;
+  constexpr bool PF_LDS = UpdShape<S>::PF_LDS;
   __shared__ double sP[NQ * NQ];
-  __shared__ double sPf[NFN * NQ];
+  __shared__ double sPf[PF_LDS ? NFN * NQ : 1];
   __shared__ double bufA[EPB][NFLD][NQ];
   __shared__ double bufB[EPB][NFLD][NQ];
   for (int idx = threadIdx.x; idx < NQ * NQ; idx += blockDim.x) sP[idx] = VqPq[idx];
-  for (int idx = threadIdx.x; idx < NFN * NQ; idx += blockDim.x) sPf[idx] = VfPq[idx];
+  if constexpr (PF_LDS)
+    for (int idx = threadIdx.x; idx < NFN * NQ; idx += blockDim.x) sPf[idx] = VfPq[idx];
 
   const int wave = threadIdx.x / WAVE;
   const int wl = threadIdx.x % WAVE;
@@ -750,7 +757,9 @@ update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ V
       for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
 #pragma unroll 2
       for (int j = 0; j < NQ; ++j) {
-        const double P = sPf[fn + j * NFN];
+        double P;
+        if constexpr (PF_LDS) P = sPf[fn + j * NFN];
+        else P = ld_off(VfPq, (unsigned)(fn + j * NFN));
         for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[e][f][j];
       }
       conservative_vars<DIM>(V, gamma, lgm1, U);
