@@ -559,26 +559,34 @@ vol_kernel(int K, const double *__restrict__ vgeo, const double *__restrict__ vf
 
 // ------------------------------------------------------------------ surface kernel
 
-template <class S, int TS, int TPW>
+// Elements per surf_kernel workgroup: the face points (then the lift rows) of
+// all of them are spread over the workgroup's 256 lanes, so no lane idles for
+// the shape of one element (3D N = 3: 48 face points and 23 rows per element
+// left 16 and 41 lanes of a wave idle).  256 / gcd(256, Nf) makes the face
+// points fill whole passes; at least 16, at most 64 (LDS: 41 KB at 3D N = 4).
+constexpr int surf_epb(int nfn) {
+  int a = 256, b = nfn;
+  while (b) { const int t = a % b; a = b; b = t; }
+  const int e = 256 / a;
+  return e < 16 ? 16 : (e > 64 ? 64 : e);
+}
+
+template <class S, int EPB>
 __global__ void __launch_bounds__(WAVE * 4)
 surf_kernel(int K, const double *__restrict__ fgeo, const uint32_t *__restrict__ mapPq,
             const double *__restrict__ VqLq, const double *__restrict__ Qf,
             const double *__restrict__ rhsQf, double *__restrict__ rhsQ, double gamma,
             double gm1, double inv2gm1, double ltau) {
   constexpr int DIM = S::DIM, NQ = S::NQ, NFN = S::NFN, NFLD = S::NFLD, NFG = DIM + 1;
-  constexpr int EPB = 4 * TPW;
+  constexpr int NT = WAVE * 4;
   __shared__ double sfS[EPB][NFLD][NFN];
-  const int wave = threadIdx.x / WAVE;
-  const int wl = threadIdx.x % WAVE;
-  const int tw = wl / TS;
-  const int lane = wl - tw * TS;
-  const bool active = tw < TPW;
-  const int e = wave * TPW + (active ? tw : 0);
-  const int k = blockIdx.x * EPB + e;
-  const bool valid = active && k < K;
+  const int k0 = blockIdx.x * EPB;
+  const int ne = (K - k0 < EPB) ? K - k0 : EPB;   // elements of this workgroup, >= 1
 
-  if (valid && lane < NFN) {
-    const int fn = lane;
+  for (int p = threadIdx.x; p < ne * NFN; p += NT) {
+    const int e = p / NFN;
+    const int fn = p - e * NFN;
+    const int k = k0 + e;
     const size_t idM = (size_t)fn + (size_t)k * NFN * NFLD;
     const size_t idP = mapPq[(size_t)k * NFN + fn];
     double UM[NFLD], UP[NFLD];
@@ -607,18 +615,19 @@ surf_kernel(int K, const double *__restrict__ fgeo, const uint32_t *__restrict__
       sfS[e][f][fn] = 0.5 * Fn[f] + rf[fn + f * NFN] - Lfc * (UP[f] - UM[f]);
   }
   __syncthreads();
-  if (valid) {
-    for (int row = lane; row < NQ; row += TS) {
-      double val[NFLD];
-      for (int f = 0; f < NFLD; ++f) val[f] = 0.0;
+  for (int p = threadIdx.x; p < ne * NQ; p += NT) {
+    const int e = p / NQ;
+    const int row = p - e * NQ;
+    const int k = k0 + e;
+    double val[NFLD];
+    for (int f = 0; f < NFLD; ++f) val[f] = 0.0;
 #pragma unroll 2
-      for (int j = 0; j < NFN; ++j) {
-        const double Lij = ld_off(VqLq, (unsigned)(row + j * NQ));
-        for (int f = 0; f < NFLD; ++f) val[f] += Lij * sfS[e][f][j];
-      }
-      double *dst = rhsQ + (size_t)k * NQ * NFLD;
-      for (int f = 0; f < NFLD; ++f) dst[row + f * NQ] += val[f];
+    for (int j = 0; j < NFN; ++j) {
+      const double Lij = ld_off(VqLq, (unsigned)(row + j * NQ));
+      for (int f = 0; f < NFLD; ++f) val[f] += Lij * sfS[e][f][j];
     }
+    double *dst = rhsQ + (size_t)k * NQ * NFLD;
+    for (int f = 0; f < NFLD; ++f) dst[row + f * NQ] += val[f];
   }
 }
 
@@ -1232,15 +1241,12 @@ template <class S> int surf_impl(const wadges_plan *p, int K, const double *fgeo
                                  const uint32_t *mapPq, const double *VqLq, const double *Qf,
                                  const double *rhsQf, double *rhsQ, hipStream_t st) {
   if (K == 0) return WADGES_OK;
-  constexpr int TS = Max<S::NQ, S::NFN>::value;
-  constexpr int TPW = WAVE / TS;
-  static_assert(TPW >= 1, "face points must fit in one wave");
-  constexpr int EPB = 4 * TPW;
+  constexpr int EPB = surf_epb(S::NFN);
   const int blocks = (K + EPB - 1) / EPB;
-  surf_kernel<S, TS, TPW><<<blocks, WAVE * 4, 0, st>>>(K, fgeo, mapPq, VqLq, Qf, rhsQf, rhsQ,
-                                                       p->gamma, p->gamma - 1.0,
-                                                       1.0 / (2.0 * (p->gamma - 1.0)),
-                                                       0.25 * p->tau);
+  surf_kernel<S, EPB><<<blocks, WAVE * 4, 0, st>>>(K, fgeo, mapPq, VqLq, Qf, rhsQf, rhsQ,
+                                                   p->gamma, p->gamma - 1.0,
+                                                   1.0 / (2.0 * (p->gamma - 1.0)),
+                                                   0.25 * p->tau);
   return hip_check(hipGetLastError());
 }
 
