@@ -1,6 +1,7 @@
 """Edge cases of the C ABI on the GPU: empty launches (K = 0) and ragged ones
-(K not a multiple of the elements a workgroup holds), which must compute the
-first K elements exactly as the oracle does and leave every other element
+(K not a multiple of the elements a workgroup holds, within one workgroup and
+across several) of the volume, surface, update and fused stage calls, which must
+compute the first K elements as the oracle does and leave every other element
 untouched; an ensemble that contains an empty problem."""
 
 import dataclasses
@@ -25,7 +26,7 @@ def _geo(dev):
 
 
 @pytest.mark.parametrize("dim,N", CASES)
-@pytest.mark.parametrize("Ksub", [0, 1, 3, 7])
+@pytest.mark.parametrize("Ksub", [0, 1, 3, 7, 17, 29])
 def test_ragged_and_empty_launches(oracle, dim, N, Ksub):
     from scratch_wadge_amd.device import DeviceSolver, _ptr
     prob = small_problem(dim, N)
@@ -52,6 +53,21 @@ def test_ragged_and_empty_launches(oracle, dim, N, Ksub):
     assert np.array_equal(rf.cpu().numpy()[Ksub * nfc:], rf_o[Ksub * nfc:])
     if Ksub:
         assert relerr(rg[:Ksub * nq], r_o[:Ksub * nq]) < 1e-12
+    # update on the first Ksub elements
+    rng = np.random.default_rng(5)
+    st = {"Q": Q.copy(), "resQ": 0.01 * rng.standard_normal(Q.shape),
+          "rhsQ": 0.1 * rng.standard_normal(rq.shape), "Qf": qf.copy()}
+    dev.set_state(**st)
+    s = dev.s
+    dev.plan.update(Ksub, g[13], g[11], g[12], -0.41, 0.37, 1e-3, _ptr(s["rhsQ"]), _ptr(s["resQ"]),
+                    _ptr(s["Q"]), _ptr(s["Qf"]), dev.stream())
+    got = dev.get_state()
+    ref = {k: v.copy() for k, v in st.items()}
+    oracle.update(sub, -0.41, 0.37, 1e-3, ref["rhsQ"], ref["resQ"], ref["Q"], ref["Qf"])
+    for k, n in (("Q", nq), ("resQ", nq), ("rhsQ", nq), ("Qf", nfc)):
+        assert np.array_equal(got[k][Ksub * n:], st[k][Ksub * n:]), k
+        if Ksub:
+            assert relerr(got[k][:Ksub * n], ref[k][:Ksub * n]) < 1e-12, k
     # fused stage on the first Ksub elements
     st = {"Q": Q.copy(), "resQ": 1e-3 * np.ones_like(Q), "rhsQ": rq.copy(), "Qf": qf.copy()}
     dev.set_state(**st)
