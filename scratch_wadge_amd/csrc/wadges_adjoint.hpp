@@ -351,8 +351,8 @@ rhs_vjp_kernel(int K, const double *__restrict__ vgeo, const double *__restrict_
 // Workgroup shape of the update VJP.  2D: one wave (its barriers cost nothing),
 // the small operators read through L1.  3D: four waves sharing the operators
 // staged in LDS (the transposed products read them along rows, which through
-// L1 touches a cache line per lane).  3D N = 3 declares three waves per SIMD
-// (168 registers instead of the 172 it takes unconstrained, which leave two).
+// L1 touches a cache line per lane).  3D N = 3 declares three waves per SIMD,
+// as many as its LDS allows (43 KB per four-wave workgroup).
 template <class S> constexpr int uvjp_wg_waves() { return S::DIM == 2 ? 1 : 4; }
 template <class S> constexpr int update_vjp_waves() { return (S::DIM == 3 && S::NQ == 23) ? 3 : 1; }
 
@@ -436,13 +436,15 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
   }
   __syncthreads();
   // cotangents of v~ through u(v~) and u(VfPq v~): the stage writes rhsQ = u(v~)
-  // (v~ already lies in the range of the projection VqPq)
-  double gq[NFLD], gfl[MAXF][NFLD];
+  // (v~ already lies in the range of the projection VqPq).  This phase reads
+  // only bufA; bufB was last read before the barrier above and bufF is read
+  // after the next one, so the results go straight to LDS.
   if (valid) {
-    double V[NFLD], Ub[NFLD];
+    double V[NFLD], Ub[NFLD], g[NFLD];
     for (int f = 0; f < NFLD; ++f) V[f] = bufA[e][f][i];
     for (int f = 0; f < NFLD; ++f) Ub[f] = uqb[base + i + f * NQ];
-    conservative_vars_vjp<DIM>(V, gamma, lgm1, Ub, gq);
+    conservative_vars_vjp<DIM>(V, gamma, lgm1, Ub, g);
+    for (int f = 0; f < NFLD; ++f) bufB[e][f][i] = g[f];
     for (int m = 0; m < MAXF; ++m) {
       const int fn = i + m * NQ;
       if (fn >= NFN) continue;
@@ -452,16 +454,8 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
         for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[e][f][j];
       }
       for (int f = 0; f < NFLD; ++f) Ub[f] = ufb[basef + fn + f * NFN];
-      conservative_vars_vjp<DIM>(V, gamma, lgm1, Ub, gfl[m]);
-    }
-  }
-  __syncthreads();
-  if (valid) {
-    for (int f = 0; f < NFLD; ++f) bufB[e][f][i] = gq[f];
-    for (int m = 0; m < MAXF; ++m) {
-      const int fn = i + m * NQ;
-      if (fn >= NFN) continue;
-      for (int f = 0; f < NFLD; ++f) bufF[e][f][fn] = gfl[m][f];
+      conservative_vars_vjp<DIM>(V, gamma, lgm1, Ub, g);
+      for (int f = 0; f < NFLD; ++f) bufF[e][f][fn] = g[f];
     }
   }
   __syncthreads();
