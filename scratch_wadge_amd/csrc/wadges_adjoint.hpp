@@ -368,7 +368,6 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
   constexpr int WPB = uvjp_wg_waves<S>();
   constexpr bool OPS_LDS = WPB > 1;
   constexpr int EPB = WPB * TPW;
-  constexpr int MAXF = (NFN + NQ - 1) / NQ;
   __shared__ double sP[OPS_LDS ? NQ * NQ : 1];
   __shared__ double sPf[OPS_LDS ? NFN * NQ : 1];
   __shared__ double bufA[EPB][NFLD][NQ];
@@ -393,7 +392,6 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
   const int k = blockIdx.x * EPB + e;
   const bool valid = active && k < K;
   const size_t base = (valid ? (size_t)k : 0) * NQ * NFLD;
-  const size_t basef = (valid ? (size_t)k : 0) * NFN * NFLD;
 
   // A x (row i) and A^T x (column i) with A = VqPq
   auto mv = [&](double (*in)[NQ], double (&r)[NFLD]) {
@@ -445,18 +443,23 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
     for (int f = 0; f < NFLD; ++f) Ub[f] = uqb[base + i + f * NQ];
     conservative_vars_vjp<DIM>(V, gamma, lgm1, Ub, g);
     for (int f = 0; f < NFLD; ++f) bufB[e][f][i] = g[f];
-    for (int m = 0; m < MAXF; ++m) {
-      const int fn = i + m * NQ;
-      if (fn >= NFN) continue;
-      for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
-      for (int j = 0; j < NQ; ++j) {
-        const double P = Pf(fn + j * NFN);
-        for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[e][f][j];
-      }
-      for (int f = 0; f < NFLD; ++f) Ub[f] = ufb[basef + fn + f * NFN];
-      conservative_vars_vjp<DIM>(V, gamma, lgm1, Ub, g);
-      for (int f = 0; f < NFLD; ++f) bufF[e][f][fn] = g[f];
+  }
+  // face points of the wave's TPW elements, spread over all its lanes
+  for (int p = wl; p < TPW * NFN; p += WAVE) {
+    const int t = p / NFN;
+    const int fn = p - t * NFN;
+    const int et = wave * TPW + t;
+    const int kt = blockIdx.x * EPB + et;
+    if (kt >= K) continue;
+    double V[NFLD], Ub[NFLD], g[NFLD];
+    for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
+    for (int j = 0; j < NQ; ++j) {
+      const double P = Pf(fn + j * NFN);
+      for (int f = 0; f < NFLD; ++f) V[f] += P * bufA[et][f][j];
     }
+    for (int f = 0; f < NFLD; ++f) Ub[f] = ufb[(size_t)kt * NFN * NFLD + fn + f * NFN];
+    conservative_vars_vjp<DIM>(V, gamma, lgm1, Ub, g);
+    for (int f = 0; f < NFLD; ++f) bufF[et][f][fn] = g[f];
   }
   __syncthreads();
   // v~_bar = g_q + VfPq^T g_f ; then through VqPq diag(1/J) VqPq
@@ -517,19 +520,20 @@ update_vjp_kernel(int K, const double *__restrict__ Jq, const double *__restrict
   if (valid)
     for (int f = 0; f < NFLD; ++f) bufA[e][f][i] = rbar[f];
   __syncthreads();
-  if (valid) {
-    for (int m = 0; m < MAXF; ++m) {
-      const int fn = i + m * NQ;
-      if (fn >= NFN) continue;
-      double v[NFLD];
-      for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
+  for (int p = wl; p < TPW * NFN; p += WAVE) {
+    const int t = p / NFN;
+    const int fn = p - t * NFN;
+    const int et = wave * TPW + t;
+    const int kt = blockIdx.x * EPB + et;
+    if (kt >= K) continue;
+    double v[NFLD];
+    for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
 #pragma unroll 4
-      for (int j = 0; j < NQ; ++j) {
-        const double L = ld_off(VqLq, (unsigned)(j + fn * NQ));
-        for (int f = 0; f < NFLD; ++f) v[f] += L * bufA[e][f][j];
-      }
-      for (int f = 0; f < NFLD; ++f) sbar[basef + f * NFN + fn] = v[f];
+    for (int j = 0; j < NQ; ++j) {
+      const double L = ld_off(VqLq, (unsigned)(j + fn * NQ));
+      for (int f = 0; f < NFLD; ++f) v[f] += L * bufA[et][f][j];
     }
+    for (int f = 0; f < NFLD; ++f) sbar[(size_t)kt * NFN * NFLD + f * NFN + fn] = v[f];
   }
 }
 
