@@ -112,17 +112,20 @@ def cpu_baseline(prob, target_s):
                       f"({dt:.1f} s, C++ oracle, OpenMP {threads} threads)"}
 
 
-def profiled_traffic(kernel, workload_file):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
-    (FETCH_SIZE / WRITE_SIZE passes scaled by the copy-kernel calibration), or
-    None when no summary covers this workload."""
+def profiled_value(key, kernel, workload_file):
+    """A per-launch figure of `kernel` recorded in a committed rocprofv3 summary
+    under profiles/ (lines "key kernel value"), or None when no summary covers
+    this workload.  Keys: traffic_bytes_per_launch (FETCH_SIZE / WRITE_SIZE
+    passes scaled by the copy-kernel calibration), fp64_lane_flops_per_launch
+    (64 (ADD + MUL + TRANS) + 128 FMA from the SQ_INSTS_VALU_*_F64 pass),
+    kernel_avg_us (the kernel-trace average)."""
     path = os.path.join(ROOT, "profiles", workload_file)
     if not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-            return int(parts[2])
+        if len(parts) == 3 and parts[0] == key and parts[1] == kernel:
+            return float(parts[2])
     return None
 
 
@@ -156,6 +159,23 @@ def ensemble_bench(steps, seed=0):
     base_g.run(2, use_graph=True)                 # capture outside the timing
     t_base_g = _sync_time(lambda: base_g.run(steps, use_graph=True))
     work = ens.dofs * 5 * steps / 1e9
+    # roofline of the ensemble kernel: one launch per stage over all problems
+    import torch
+    from scratch_wadge_amd import workmodel as W
+    stream = torch.cuda.current_stream()
+    evs = []
+    for _ in range(4):
+        for s in range(5):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            ens.stage(s)
+            b.record(stream)
+            evs.append((a, b))
+    torch.cuda.synchronize()
+    launch_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    flops = sum(p.K * W.stage_flops(2, p.Nq, p.Nf) for p in probs)
+    achieved = flops / (launch_ms * 1e-3) / 1e12
     g = lambda t: round(work / t, 4)
     return {"workload": "64 2D isentropic vortex problems, K log-uniform in [128, 8192] "
                         "(nearest 2n^2), N in {2,3,4}, strength U[3,6], centre U[-2,2]^2, seed 0",
@@ -165,7 +185,11 @@ def ensemble_bench(steps, seed=0):
             "speedup_vs_per_problem_eager": round(t_base / t_batch, 3),
             "speedup_vs_per_problem_graph": round(t_base_g / t_batch, 3),
             "launch_shape": ens.config(), "tuned_stage_ms": round(tuned_ms, 4),
-            "bitwise_equal_to_per_problem": bool(equal), "finite": bool(finite)}
+            "bitwise_equal_to_per_problem": bool(equal), "finite": bool(finite),
+            "roofline": {"bound": "fp64 valu", "kernel": "batch_stage_flat",
+                         "achieved": round(achieved, 3), "peak": W.FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / W.FP64_PEAK_TFLOPS, 4),
+                         "flops_per_launch": flops, "launch_ms": round(launch_ms, 4)}}
 
 
 def adjoint3d_bench(steps=8, ckpt=4, n=32):
@@ -227,6 +251,46 @@ def adjoint_bench(steps, ckpt=40):
             "adjoint_over_forward": round(t_bwd / t_plain, 3),
             "forward_plus_adjoint_over_forward": round((t_fwd + t_bwd) / t_plain, 3),
             "loss": loss, "grad_norm": float(torch.linalg.norm(out["g"]))}
+
+
+def n4_leg(n, steps=3):
+    """The headline mesh at N = 4 (Np = 35, Nq = 44, Nf = 64): fused-stage launch
+    time (HIP events, 5 steps after one warm-up step) and GDOF.stage/s."""
+    import numpy as np
+    import torch
+    from scratch_wadge_amd import problem as P
+    from scratch_wadge_amd import workmodel as W
+    from scratch_wadge_amd.device import DeviceSolver
+    prob = P.tgv3d_problem(n=n, N=4, curved=True)
+    dev = DeviceSolver(prob)
+    dev.step(prob.dt, fused=True)
+    stream = torch.cuda.current_stream()
+    evs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in range(5):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            dev.stage_fused(s, prob.dt)
+            b.record(stream)
+            evs.append((a, b))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    flops = W.stage_flops(3, prob.Nq, prob.Nf) * prob.K
+    ok = bool(np.isfinite(dev.s["Q"].cpu().numpy()).all())
+    del dev
+    torch.cuda.empty_cache()
+    return {"workload": f"3D TGV curved (warp 0.1) tets, {n}^3 x 6 = {prob.K} elements, N=4",
+            "value": round(W.dofs(3, 4, prob.K) * 5 * steps / wall / 1e9, 4),
+            "unit": "GDOF*stage/s", "stage_ms": round(ms, 4),
+            "roofline": {"bound": "fp64 valu", "kernel": "stage_kernel",
+                         "achieved": round(flops / (ms * 1e-3) / 1e12, 3),
+                         "frac": round(flops / (ms * 1e-3) / 1e12 / W.FP64_PEAK_TFLOPS, 4),
+                         "flops_per_element": W.stage_flops(3, prob.Nq, prob.Nf)},
+            "finite": ok}
 
 
 def main():
@@ -306,10 +370,39 @@ def main():
 
     stages = 5 * args.steps
     value = W.dofs(prob.dim, prob.N, prob.K) * stages / elapsed / 1e9
+    prof_file = (f"stage_kernel_tgv3d_{'affine' if args.affine else 'curved'}"
+                 f"_K{prob.K}_N{prob.N}.md")
+    prof = lambda key, kernel: profiled_value(key, kernel, prof_file)
+
+    def counter_tflops(kernel, ms):
+        v = prof("fp64_lane_flops_per_launch", kernel)
+        return None if v is None else round(v / (ms * 1e-3) / 1e12, 3)
+
+    def prof_ms(kernel):
+        v = prof("kernel_avg_us", kernel)
+        return None if v is None else round(v / 1e3, 4)
     flops = W.stage_flops(prob.dim, prob.Nq, prob.Nf) * prob.K
     achieved = flops / (stage_ms * 1e-3) / 1e12
     vflops = W.vol_flops(prob.dim, prob.Nq, prob.Nf) * prob.K
     st = dev.get_state()
+    varying_ms = None
+    if args.extras:
+        # ---- the same stage on a strongly varying state, where neighbouring nodes
+        # differ beyond the log mean's series threshold (both branches taken); the
+        # Taylor-Green state above takes the series branch throughout
+        Qv, rqv, qfv = P.varying_state(prob, seed=7, amp=0.3)
+        dev.set_state(Q=Qv, rhsQ=rqv, Qf=qfv, resQ=np.zeros_like(Qv))
+        evr = []
+        for _ in range(3):
+            for s in range(5):
+                a = ev()
+                dev.stage_fused(s, dt)
+                evr.append((a, ev()))
+        torch.cuda.synchronize()
+        varying_ms = float(np.mean([a.elapsed_time(b) for a, b in evr[5:]]))
+        if not np.isfinite(dev.s["Q"].cpu().numpy()).all():
+            print("# varying state is not finite after its steps", file=sys.stderr)
+            sys.exit(1)
     if not all(np.isfinite(v).all() for v in st.values()):
         print("# state is not finite after the run", file=sys.stderr)
         sys.exit(1)
@@ -335,19 +428,30 @@ def main():
         "kernels_ms": {"stage_fused": round(stage_ms, 4), "split_stage": round(split_stage_ms, 4),
                        "vol": round(vol_ms, 4), "surf": round(surf_ms, 4), "update": round(upd_ms, 4)},
         "roofline": {
-            "bound": "mfma", "pipe": "fp64 vector ALU (its published peak equals the FP64 matrix peak)",
+            "bound": "fp64 valu",
+            "pipe": "FP64 vector ALU: the flux differencing is elementwise D o F over "
+                    "node pairs, not a matrix product, and issues no MFMA",
             "kernel": "stage_kernel",
             "achieved": round(achieved, 3), "peak": W.FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / W.FP64_PEAK_TFLOPS, 4),
             "flops_per_launch": flops,
-            "traffic": profiled_traffic(
-                "stage_kernel", f"stage_kernel_tgv3d_{'affine' if args.affine else 'curved'}"
-                                f"_K{prob.K}_N{prob.N}.md"),
+            "flops_per_element": W.stage_flops(prob.dim, prob.Nq, prob.Nf),
+            "launch_ms": round(stage_ms, 4),
+            "traffic": prof("traffic_bytes_per_launch", "stage_kernel"),
             "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (copy-calibrated), "
-                              "profiles/stage_kernel_tgv3d_*.md",
+                              f"profiles/{prof_file}",
+            "counter_tflops": counter_tflops("stage_kernel", stage_ms),
+            "counter_source": "64 (ADD+MUL+TRANS) + 128 FMA FP64 instructions per launch "
+                              f"(SQ_INSTS_VALU_*_F64), profiles/{prof_file}, over launch_ms",
+            "profile_kernel_avg_ms": prof_ms("stage_kernel"),
         },
-        "vol_kernel_roofline": {"achieved": round(vflops / (vol_ms * 1e-3) / 1e12, 3),
-                                "frac": round(vflops / (vol_ms * 1e-3) / 1e12 / W.FP64_PEAK_TFLOPS, 4)},
+        "vol_kernel_roofline": {"bound": "fp64 valu", "kernel": "vol_kernel",
+                                "achieved": round(vflops / (vol_ms * 1e-3) / 1e12, 3),
+                                "frac": round(vflops / (vol_ms * 1e-3) / 1e12 / W.FP64_PEAK_TFLOPS, 4),
+                                "flops_per_element": W.vol_flops(prob.dim, prob.Nq, prob.Nf),
+                                "launch_ms": round(vol_ms, 4),
+                                "counter_tflops": counter_tflops("vol_kernel", vol_ms),
+                                "profile_kernel_avg_ms": prof_ms("vol_kernel")},
         "hbm_GBs": {
             "stage_fused": round(W.bytes_stage_fused(prob.dim, prob.Nq, prob.Nf) * prob.K / (stage_ms * 1e-3) / 1e9, 1),
             "surf": round(W.bytes_surf(prob.dim, prob.Nq, prob.Nf) * prob.K / (surf_ms * 1e-3) / 1e9, 1),
@@ -357,8 +461,22 @@ def main():
     }
     if args.cpu:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
-    if args.extras:
+    if varying_ms is not None:
+        out["varying_state"] = {
+            "state": "density and pressure 1 + 0.3 sin(x.k + phase), velocities 0.5: both "
+                     "log-mean branches (the Taylor-Green state takes the series branch at "
+                     "every pair)",
+            "stage_ms": round(varying_ms, 4),
+            "value": round(W.dofs(prob.dim, prob.N, prob.K) / (varying_ms * 1e-3) / 1e9, 4),
+            "frac": round(flops / (varying_ms * 1e-3) / 1e12 / W.FP64_PEAK_TFLOPS, 4)}
+    if args.extras and args.N == 3 and not args.affine:
         del dev
+        torch.cuda.empty_cache()
+        out["N4"] = n4_leg(args.n)
+        print(f"# N4 {out['N4']}", file=sys.stderr, flush=True)
+    if args.extras:
+        if "dev" in locals():
+            del dev
         torch.cuda.empty_cache()
         out["ensemble"] = ensemble_bench(args.ensemble_steps)
         print(f"# ensemble {out['ensemble']}", file=sys.stderr, flush=True)

@@ -23,9 +23,13 @@
  * operators through tables the plan re-lays out on the device.  The tables are
  * prepared on the first such call, and again whenever the operator pointers
  * (Drq, Dsq, Dtq, VqLq, VfPq) differ from those of the previous preparation;
- * that call waits for the preparation (outside stream capture; under capture
- * the preparation is recorded into the graph).  After changing operator values
- * in place, call wadges_plan_invalidate.
+ * that call waits for the preparation.  Captured calls (HIP stream capture)
+ * read a second set of tables, whose preparation is recorded into the graph
+ * at the capture's first call with those operators: a graph rewrites the
+ * tables it reads whenever it runs and never touches those of the calls
+ * outside capture.  Graphs of one plan therefore must not run concurrently
+ * with each other.  After changing operator values in place, call
+ * wadges_plan_invalidate.
  *
  * Layouts (Appendix of the reference's operators_c.cpp:322-389 and
  * euler.c:3158-3170), all FP64 except mapPq:

@@ -141,8 +141,8 @@ def stage(prob, st, fa, fb, dt):
 
 
 def flop_counts(dim, Nq, Nfq):
-    out = np.zeros(12)
+    out = np.zeros(13)
     lib().wo_flop_counts(dim, Nq, Nfq, _p(out))
     keys = ["c_F", "c_node", "c_vv", "c_vf", "vol_elem", "surf_elem", "update_elem",
-            "F_log", "F_exp", "F_pow", "F_sqrt", "F_div"]
+            "F_log", "F_exp", "F_pow", "F_sqrt", "F_div", "vf_pair_deviation"]
     return dict(zip(keys, out.tolist()))

@@ -297,6 +297,32 @@ inline void contract_vf(const T *g1, const T *g2, const T *nref, T w, const T F[
   }
 }
 
+// One volume-face pair with the pair's work shared by its two coupling blocks:
+// the geofac average, the averaged normal n = {G} nref and F.n are formed once
+// and weighted into the volume row (w1 = +1/2 VqLq) and the face row
+// (w2 = -1/2 VfPq).  Equal to contract_vf for both blocks up to rounding; this
+// is the form whose operation count is the algorithmic work of a pair.
+template <int DIM, typename T>
+inline void contract_vf_pair(const T *gv, const T *gf, const T *nref, T w1, T w2,
+                             const T F[DIM][DIM + 2], T *divV, T *divF) {
+  constexpr int NF = DIM + 2;
+  T Ga[DIM][DIM];
+  for (int r = 0; r < DIM; ++r)
+    for (int d = 0; d < DIM; ++d) Ga[r][d] = T(0.5) * (gv[r * DIM + d] + gf[r * DIM + d]);
+  T n[DIM];
+  for (int d = 0; d < DIM; ++d) {
+    T v = Ga[0][d] * nref[0];
+    for (int r = 1; r < DIM; ++r) v = v + Ga[r][d] * nref[r];
+    n[d] = v;
+  }
+  for (int f = 0; f < NF; ++f) {
+    T Fn = F[0][f] * n[0];
+    for (int d = 1; d < DIM; ++d) Fn = Fn + F[d][f] * n[d];
+    divV[f] += w1 * Fn;
+    divF[f] += w2 * Fn;
+  }
+}
+
 // Volume kernel for one element: rhsQ (vol rows) and rhsQf (face rows),
 // okl/Euler3D.okl:2123-2523 / okl/Euler2D.okl:1112-1350.
 template <int DIM, typename T>
@@ -658,11 +684,13 @@ int wo_update(const wo_params *p, int K, const double *Jq, const double *VqPq, c
 //   out[0] c_F   : one two-point flux evaluation from primitive states
 //   out[1] c_node: one conservative->primitive conversion (per volume/face node)
 //   out[2] c_vv  : contraction work of one ordered volume-volume pair
-//   out[3] c_vf  : contraction work of one volume-face pair, both coupling blocks
+//   out[3] c_vf  : contraction work of one volume-face pair, both coupling blocks,
+//                  the average, normal and F.n shared by the blocks (contract_vf_pair)
 //   out[4] vol   : whole reference-structured volume kernel for one element
 //   out[5] surf  : whole surface kernel for one element
 //   out[6] update: whole update kernel for one element
-// plus transcendental splits out[7..] = {log, exp, pow, sqrt, div} of c_F.
+// plus transcendental splits out[7..] = {log, exp, pow, sqrt, div} of c_F, and
+// out[12] = max |contract_vf_pair - contract_vf of both blocks| on that pair.
 void wo_flop_counts(int dim, int Nq, int Nfq, double *out) {
   const Sizes S = make_sizes(dim, Nq, Nfq);
   const double gamma = 1.4;
@@ -698,12 +726,20 @@ void wo_flop_counts(int dim, int Nq, int Nfq, double *out) {
       g_tally.reset();
       contract_vv<DIM>(g1, g2, D, F, divF);
       out[2] = (double)g_tally.total();
+      // both blocks of one pair, the pair work counted once (contract_vf_pair);
+      // out[12]: its largest deviation from the two per-block contractions
+      const T w1 = T(0.5) * T(0.2), w2 = T(-0.5) * T(0.3);
+      T dV[NF], dF[NF], eV[NF], eF[NF];
+      for (int f = 0; f < NF; ++f) dV[f] = dF[f] = eV[f] = eF[f] = 0.0;
       g_tally.reset();
-      const T w1 = T(0.5) * T(0.2);
-      contract_vf<DIM>(g1, g2, nref, w1, F, divF);
-      const T w2 = T(-0.5) * T(0.2);
-      contract_vf<DIM>(g2, g1, nref, w2, F, divF);
+      contract_vf_pair<DIM>(g1, g2, nref, w1, w2, F, dV, dF);
       out[3] = (double)g_tally.total();
+      contract_vf<DIM>(g1, g2, nref, w1, F, eV);
+      contract_vf<DIM>(g2, g1, nref, w2, F, eF);
+      double dev = 0.0;
+      for (int f = 0; f < NF; ++f)
+        dev = std::max(dev, std::max(std::fabs(dV[f].v - eV[f].v), std::fabs(dF[f].v - eF[f].v)));
+      out[12] = dev;
     }
     // one element of each kernel with synthetic data (values only steer branches)
     const int NG = DIM * DIM;

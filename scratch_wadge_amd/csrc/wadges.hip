@@ -1186,32 +1186,50 @@ struct Ops {
 struct wadges_plan {
   int dim, N, Nq, Nfq, Nfaces, Nf, nfields;
   double gamma, tau;
-  double *ws;  // device workspace (operator tables)
+  double *ws;  // device workspace: two sets of operator tables
   int device;
   wadges::Ops ops;
-  // Operator pointers the tables in ws were prepared from (null: not prepared).
+  // Operator pointers the tables of the calls outside capture (ws[0, WS)) were
+  // prepared from (null: not prepared).
   mutable const double *tab_src[5];
+  // The capture (its id) that last recorded a preparation of the captured
+  // tables (ws[WS, 2 WS)), and from which operator pointers.
+  mutable unsigned long long cap_id;
+  mutable const double *cap_src[5];
 };
 
 namespace wadges {
 
 // Prepares the operator tables of the plan unless they were prepared from the
-// same operator pointers before.  Outside stream capture a fresh preparation
-// is waited for, so the tables are then valid for calls on every stream; under
-// capture a needed preparation is recorded into the graph and the cache is
-// left alone (the tables become valid only when the graph runs).
+// same operator pointers before, and returns in *tab the tables the kernels of
+// this call read.  The plan holds two sets: one for calls outside stream
+// capture and one for captured calls.  Outside capture a fresh preparation is
+// waited for, so the tables are then valid for calls on every stream.  Under
+// capture the preparation of the captured set is recorded into the graph at
+// the capture's first call with these operators: every graph rewrites the
+// tables it reads whenever it runs, and no graph touches the tables of the
+// calls outside capture.
 template <class S>
 int ensure_tables(const wadges_plan *p, const double *Drq, const double *Dsq, const double *Dtq,
-                  const double *VqLq, const double *VfPq, hipStream_t st) {
+                  const double *VqLq, const double *VfPq, hipStream_t st, const double **tab) {
   const double *src[5] = {Drq, Dsq, Dtq, VqLq, VfPq};
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  int rc = hip_check(hipStreamIsCapturing(st, &cs));
+  unsigned long long id = 0;
+  int rc = hip_check(hipStreamGetCaptureInfo(st, &cs, &id));
   if (rc) return rc;
   const bool capturing = cs != hipStreamCaptureStatusNone;
-  if (std::equal(src, src + 5, p->tab_src)) return WADGES_OK;
-  prep_vol_tables<S><<<(S::WS + 255) / 256, 256, 0, st>>>(Drq, Dsq, Dtq, VqLq, VfPq, p->ws);
+  double *ws = capturing ? p->ws + S::WS : p->ws;
+  *tab = ws;
+  if (capturing ? (id == p->cap_id && std::equal(src, src + 5, p->cap_src))
+                : std::equal(src, src + 5, p->tab_src))
+    return WADGES_OK;
+  prep_vol_tables<S><<<(S::WS + 255) / 256, 256, 0, st>>>(Drq, Dsq, Dtq, VqLq, VfPq, ws);
   if ((rc = hip_check(hipGetLastError()))) return rc;
-  if (capturing) return WADGES_OK;
+  if (capturing) {
+    p->cap_id = id;
+    std::copy(src, src + 5, p->cap_src);
+    return WADGES_OK;
+  }
   if ((rc = hip_check(hipStreamSynchronize(st)))) return rc;
   std::copy(src, src + 5, p->tab_src);
   return WADGES_OK;
@@ -1223,15 +1241,16 @@ template <class S> int vol_impl(const wadges_plan *p, int K, const double *vgeo,
                                 const double *Dtq, const double *VqLq, const double *VfPq,
                                 const double *Qf, double *rhsQ, double *rhsQf, hipStream_t st) {
   if (K == 0) return WADGES_OK;
+  const double *tab = nullptr;
   {
-    int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
+    int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st, &tab);
     if (rc) return rc;
   }
   constexpr int TPW = S::TPW;
   static_assert(TPW >= 1, "Nq must fit in one wave");
   const size_t smem = (size_t)TPW * VolSmem<S>::per_elem * sizeof(double);
   const int blocks = (K + TPW - 1) / TPW;
-  vol_kernel<S><<<blocks, WAVE, smem, st>>>(K, vgeo, vfgeo, nrJ, nsJ, ntJ, p->ws, Qf, rhsQ,
+  vol_kernel<S><<<blocks, WAVE, smem, st>>>(K, vgeo, vfgeo, nrJ, nsJ, ntJ, tab, Qf, rhsQ,
                                                  rhsQf, p->gamma - 1.0,
                                                  1.0 / (2.0 * (p->gamma - 1.0)));
   return hip_check(hipGetLastError());
@@ -1272,14 +1291,15 @@ template <class S> int stage_impl(const wadges_plan *p, int K, const double *vge
                                   double *rhsQ, const double *resQ_in, double *resQ,
                                   const double *Q_in, double *Q, hipStream_t st) {
   if (K == 0) return WADGES_OK;
-  int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
+  const double *tab = nullptr;
+  int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st, &tab);
   if (rc) return rc;
   constexpr int TPW = S::TPW;
   const size_t smem = (size_t)TPW * VolSmem<S>::per_elem * sizeof(double);
   const int blocks = (K + TPW - 1) / TPW;
   const double g = p->gamma;
   StageArgs A{K,      vgeo,   vfgeo,   fgeo,   mapPq,  nrJ,     nsJ,  ntJ,
-              p->ws,  VqLq,   VqPq,    VfPq,   Jq,     fa,      fb,   fdt,
+              tab,    VqLq,   VqPq,    VfPq,   Jq,     fa,      fb,   fdt,
               Qf_in,  Qf_out, rhsQ,    resQ,   Q,      rhsQ_in, resQ_in, Q_in,
               g,      g - 1.0, 1.0 / (2.0 * (g - 1.0)), 0.25 * p->tau, std::log(g - 1.0)};
   stage_kernel<S><<<blocks, WAVE, smem, st>>>(A);
@@ -1307,7 +1327,8 @@ template <class S> int rhs_vjp_impl(const wadges_plan *p, int K, const double *v
                                     const double *Uf, const double *Rb, double *sbar, double *uqb,
                                     double *ufb, hipStream_t st) {
   if (K == 0) return WADGES_OK;
-  int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
+  const double *tab = nullptr;
+  int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st, &tab);
   if (rc) return rc;
   {
     constexpr int TS = Max<S::NQ, S::NFN>::value;
@@ -1321,7 +1342,7 @@ template <class S> int rhs_vjp_impl(const wadges_plan *p, int K, const double *v
   const size_t smem = (size_t)TPW * VjpSmem<S>::per_elem * sizeof(double);
   const double g = p->gamma;
   rhs_vjp_kernel<S, TPW><<<(K + TPW - 1) / TPW, WAVE, smem, st>>>(
-      K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, p->ws, Uq, Uf, Rb, sbar, uqb, ufb, g, g - 1.0,
+      K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, tab, Uq, Uf, Rb, sbar, uqb, ufb, g, g - 1.0,
       1.0 / (2.0 * (g - 1.0)), 0.25 * p->tau);
   return hip_check(hipGetLastError());
 }
@@ -1354,7 +1375,8 @@ template <class S> int stage_vjp_impl(const wadges_plan *p, int K, const double 
                                       double *sbar, double *uqb_in, double *ufb_in,
                                       hipStream_t st) {
   if (K == 0) return WADGES_OK;
-  int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st);
+  const double *tab = nullptr;
+  int rc = ensure_tables<S>(p, Drq, Dsq, Dtq, VqLq, VfPq, st, &tab);
   if (rc) return rc;
   const double g = p->gamma;
   {
@@ -1368,7 +1390,7 @@ template <class S> int stage_vjp_impl(const wadges_plan *p, int K, const double 
   constexpr int TPW = WAVE / S::NQ;
   const size_t smem = (size_t)TPW * VjpSmem<S>::per_elem * sizeof(double);
   rhs_vjp_kernel<S, TPW><<<(K + TPW - 1) / TPW, WAVE, smem, st>>>(
-      K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, p->ws, rhsQ_in, Qf_in, Rb, sbar, uqb_in, ufb_in,
+      K, vgeo, vfgeo, fgeo, mapPq, nrJ, nsJ, ntJ, tab, rhsQ_in, Qf_in, Rb, sbar, uqb_in, ufb_in,
       g, g - 1.0, 1.0 / (2.0 * (g - 1.0)), 0.25 * p->tau);
   return hip_check(hipGetLastError());
 }
@@ -1477,11 +1499,13 @@ int wadges_plan_create(wadges_plan **plan, int dim, int N, int Nq, int Nfq, doub
   p->ops = inst->ops;
   p->ws = nullptr;
   for (auto &t : p->tab_src) t = nullptr;
+  p->cap_id = 0;
+  for (auto &t : p->cap_src) t = nullptr;
   if (hipGetDevice(&p->device) != hipSuccess) {
     delete p;
     return WADGES_EHIP;
   }
-  hipError_t e = hipMalloc(&p->ws, sizeof(double) * (size_t)inst->ops.ws_doubles);
+  hipError_t e = hipMalloc(&p->ws, sizeof(double) * 2 * (size_t)inst->ops.ws_doubles);
   if (e != hipSuccess) {
     g_last_hip = (int)e;
     delete p;
@@ -1502,6 +1526,8 @@ int wadges_plan_destroy(wadges_plan *plan) {
 int wadges_plan_invalidate(wadges_plan *plan) {
   if (!plan) return WADGES_EINVAL;
   for (auto &t : plan->tab_src) t = nullptr;
+  plan->cap_id = 0;
+  for (auto &t : plan->cap_src) t = nullptr;
   return WADGES_OK;
 }
 

@@ -200,6 +200,33 @@ def make_problem(dim, N, mesh, ic, warp=None, gamma=GAMMA, tau=1.0, cfl=0.5, sta
         meta={"matched_face_points": nmatched, "geometry": geo, "mesh": mesh})
 
 
+def varying_state(prob: Problem, seed=0, amp=0.3):
+    """A smooth but strongly varying state on prob's mesh: density and pressure
+    1 + amp sin(k.x + phase) and velocities of 0.5 (wave numbers and phases
+    from seed), projected like the initial condition.  Neighbouring nodes
+    differ by more than the log mean's series threshold, so both branches of
+    the log mean are taken.  Returns (Q, rhsQ, Qf) in kernel layout."""
+    rng = np.random.default_rng(seed)
+    xq = prob.xq
+    dim = prob.dim
+    ph = rng.uniform(0, 2 * np.pi, size=(4, dim))
+    kv = rng.integers(1, 3 if dim == 2 else 2, size=(4, dim))
+
+    def wave(m):
+        return np.sin(sum(kv[m, d] * xq[d] * (2 * np.pi / 10.0 if dim == 2 else 1.0) + ph[m, d]
+                          for d in range(dim)))
+
+    rho = 1.0 + amp * wave(0)
+    p = 1.0 + amp * wave(1)
+    vel = [0.5 * wave(2 + (d % 2)) * (1 if d == 0 else -1) for d in range(dim)]
+    E = p / (prob.gamma - 1.0) + 0.5 * rho * sum(v * v for v in vel)
+    U = np.stack([rho] + [rho * v for v in vel] + [E], 0)
+    Q, Uq, Uf = project_initial_state(prob.ops, U, prob.gamma)
+    if not (np.isfinite(Uq).all() and np.isfinite(Uf).all()):
+        raise ValueError("the varying state left the admissible set")
+    return _to_layout(Q), _to_layout(Uq), _to_layout(Uf)
+
+
 # ------------------------------------------------------------------ named workloads
 
 def vortex2d_problem(n=16, N=3, perturb=0.05, seed=1, beta=5.0, x0=0.0, y0=0.0, tau=1.0,

@@ -12,7 +12,10 @@ constants still equal those counts):
     c_vv   averaged-metric contraction of one ordered volume-volume pair
            (okl/Euler3D.okl:2317-2340)
     c_vf   averaged-normal contraction of one volume-face pair, both coupling
-           blocks (okl/Euler3D.okl:2383-2403 and :2488-2507)
+           blocks (okl/Euler3D.okl:2383-2403 and :2488-2507), with the
+           geofac average, the normal and F.n formed once per pair and
+           weighted into both rows: 3D 18 + 15 + 25 + 4 x 5 = 78, 2D
+           8 + 6 + 12 + 4 x 4 = 42
 
 Volume kernel work counts each distinct flux once (the EC flux is symmetric):
 
@@ -30,8 +33,8 @@ from __future__ import annotations
 
 # exact counts (add+mul+div+log), see module docstring
 PIECES = {
-    3: {"c_F": 75, "c_node": 15, "c_vv": 63, "c_vf": 138},
-    2: {"c_F": 63, "c_node": 11, "c_vv": 48, "c_vf": 106},
+    3: {"c_F": 75, "c_node": 15, "c_vv": 63, "c_vf": 78},
+    2: {"c_F": 63, "c_node": 11, "c_vv": 48, "c_vf": 42},
 }
 
 # exact counts of the surface and update kernels per element (reference loop
@@ -39,9 +42,15 @@ PIECES = {
 # the Nq x Nf lift; four Nq x Nq and one Nf x Nq mat-vec per field plus the
 # variable maps), keyed by (dim, Nq, Nf)
 SURF_UPDATE = {
+    (2, 3, 6): (1098, 852),
+    (2, 6, 9): (1869, 2544),
+    (2, 12, 12): (3084, 8280),
+    (2, 16, 15): (4339, 14092),
+    (2, 25, 18): (6526, 31960),
+    (3, 4, 12): (2648, 2000),
+    (3, 11, 24): (6991, 10590),
     (3, 23, 48): (19747, 41958),
     (3, 44, 64): (39836, 134632),
-    (2, 12, 12): (3084, 8280),
 }
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak

@@ -47,10 +47,17 @@ def test_workmodel_matches_oracle_flop_counts(oracle):
         c = oracle.flop_counts(dim, Nq, Nfq)
         for k in ("c_F", "c_node", "c_vv", "c_vf"):
             assert W.PIECES[dim][k] == c[k], (dim, k)
+        # the counted pair form computes what the two per-block contractions do
+        assert c["vf_pair_deviation"] <= 1e-14
+    # one volume-face pair: average 2 NG, normal DIM (2 DIM - 1), F.n NFLD (2 DIM - 1),
+    # and 2 flops per field for each of the two rows
+    for dim in (2, 3):
+        ng, nf = dim * dim, dim + 2
+        assert W.PIECES[dim]["c_vf"] == 2 * ng + (dim + nf) * (2 * dim - 1) + 4 * nf
     for (dim, Nq, Nf), (sf, uf) in W.SURF_UPDATE.items():
         c = oracle.flop_counts(dim, Nq, Nf // (dim + 1))
         assert (sf, uf) == (c["surf_elem"], c["update_elem"]), (dim, Nq)
-    assert W.vol_flops(3, 23, 48) == 290244
+    assert W.vol_flops(3, 23, 48) == 224004
     assert W.bytes_vol(3, 23, 48) == 10792
     assert W.bytes_surf(3, 23, 48) == 9328
     assert W.bytes_update(3, 23, 48) == 7624

@@ -129,3 +129,90 @@ def test_plan_invalidate_rereads_operators():
     assert np.array_equal(r_stale, r0)
     assert np.array_equal(r1, r_ref)
     assert relerr(r1, r0) > 1e-3
+
+
+def test_config4_workload_against_oracle(oracle):
+    """The 64-problem ensemble workload itself (2D isentropic vortices, K
+    log-uniform in [128, 8192], N in {2, 3, 4}, strength U[3, 6], centre
+    U[-2, 2]^2, seed 0) with the autotuned launch shape and graph-replayed
+    steps: every problem bit for bit equal to its own single-problem stages, and
+    the smallest and the largest problem of each degree equal to the oracle run
+    through the same steps (each problem is the reference's rk_run,
+    euler.c:3672-3699, on its own mesh)."""
+    from scratch_wadge_amd.ensemble import Ensemble, PerProblem, vortex_ensemble
+    probs = vortex_ensemble(64, seed=0)
+    steps = 4
+    print(f"case config-4 ensemble: 64 problems, K_total {sum(p.K for p in probs)}, "
+          f"{steps} steps", flush=True)
+    ens, ref = Ensemble(probs), PerProblem(probs)
+    ens.tune(reps=2)
+    print("tuned launch shape", ens.config(), flush=True)
+    ens.run(steps, use_graph=True)
+    ref.run(steps)
+    for q in range(len(probs)):
+        a, b = ens.state(q), ref.state(q)
+        for f in FIELDS:
+            assert np.isfinite(a[f]).all(), (q, f)
+            assert np.array_equal(a[f], b[f]), (q, f)
+    picks = set()
+    for N in (2, 3, 4):
+        qs = [q for q, p in enumerate(probs) if p.N == N]
+        assert qs, N
+        picks.add(min(qs, key=lambda q: probs[q].K))
+        picks.add(max(qs, key=lambda q: probs[q].K))
+    for q in sorted(picks):
+        p = probs[q]
+        st = p.copy_state()
+        for _ in range(steps):
+            for s in range(5):
+                oracle.stage(p, st, P.RK4A[s], P.RK4B[s], p.dt)
+        a = ens.state(q)
+        errs = {f: relerr(a[f], st[f]) for f in ("Q", "rhsQ", "Qf")}
+        print(f"problem {q}: N={p.N} K={p.K} vs oracle " +
+              " ".join(f"{f} {e:.2e}" for f, e in errs.items()), flush=True)
+        assert max(errs.values()) <= 1e-12, (q, errs)
+
+
+def test_graph_tables_and_eager_tables_interleaved():
+    """A graph captured with one set of operators (B) and calls outside capture
+    with another (A) on the same plan, interleaved: every call reads the tables
+    of its own operators, equal bit for bit to fresh plans with A and with B."""
+    import torch
+    from scratch_wadge_amd.device import DeviceSolver
+    prob = P.tgv3d_problem(n=3, N=3, curved=True)
+    print("case graph / eager operator tables", flush=True)
+    names = ("Drq", "Dsq", "Dtq", "VqLq", "VfPq")
+    a = DeviceSolver(prob)
+    A = {k: a.c[k] for k in names}
+    B = {k: (1.5 * v).contiguous() for k, v in A.items()}      # other values, other pointers
+    ra, rfa = DeviceSolver(prob).vol_only(prob.rhsQ, prob.Qf)
+    b = DeviceSolver(prob)
+    b.c.update(B)
+    rb, rfb = b.vol_only(prob.rhsQ, prob.Qf)
+    assert relerr(ra, rb) > 1e-3
+    rq = torch.as_tensor(prob.rhsQ, device=a.device)
+    qf = torch.as_tensor(prob.Qf, device=a.device)
+    rg, rfg = rq.clone(), torch.zeros_like(qf)
+
+    def eager_a():
+        r, rf = a.vol_only(prob.rhsQ, prob.Qf)
+        assert np.array_equal(r, ra) and np.array_equal(rf, rfa)
+
+    def replay_b():
+        rg.copy_(rq)                                 # vol overwrites its input
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(rg.cpu().numpy(), rb) and np.array_equal(rfg.cpu().numpy(), rfb)
+
+    eager_a()                                        # tables of A prepared and cached
+    g = torch.cuda.CUDAGraph()
+    a.c.update(B)
+    with torch.cuda.graph(g):
+        a.vol(rg, qf, rfg)
+    a.c.update(A)
+    replay_b()
+    eager_a()
+    replay_b()
+    replay_b()
+    eager_a()
+    eager_a()

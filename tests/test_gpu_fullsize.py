@@ -89,3 +89,59 @@ def test_full_size_entropy_production(oracle, n, N, curved):
     assert rel0 < max(1e-12, 10 * rel_o)
     # du/dt = -2 M^-1 rhs, so a dissipative penalty makes sum wq v.rhs positive
     assert t1.sum() > 0 and abs(t1.sum()) > 1e3 * abs(t0.sum())
+
+
+@pytest.mark.parametrize("n,N,curved", CASES)
+def test_full_size_rough_state_flat_bar(oracle, n, N, curved):
+    """A well-conditioned state at full size: density and pressure varying by
+    30 % and velocities of 0.5 (both log-mean branches taken), where the RHS
+    is O(1) rather than a cancellation residual.  Held to a flat 1e-12
+    relative L2 against the oracle, the RHS and the face terms."""
+    from helpers import rough_state
+    prob = P.tgv3d_problem(n=n, N=N, curved=curved)
+    _, rq, qf = rough_state(prob, seed=7, amp=0.3)
+    print(f"case rough full size n={n} N={N} curved={curved} K={prob.K}", flush=True)
+    from scratch_wadge_amd.device import DeviceSolver
+    dev = DeviceSolver(prob)
+    r_g, rf_g = dev.vol_only(rq, qf)
+    r_gs, _ = dev.rhs(rq, qf)
+    del dev
+    torch.cuda.empty_cache()
+    r_o = rq.copy()
+    rf_o = np.zeros_like(qf)
+    oracle.vol(prob, r_o, qf, rf_o)
+    e_vol, e_f = relerr(r_g, r_o), relerr(rf_g, rf_o)
+    oracle.surf(prob, qf, rf_o, r_o)
+    e = relerr(r_gs, r_o)
+    print(f"rel L2 vs oracle: volume {e_vol:.2e}, face terms {e_f:.2e}, vol+surf {e:.2e}",
+          flush=True)
+    assert e_vol <= 1e-12 and e_f <= 1e-12 and e <= 1e-12
+
+
+def _oracle_steps(oracle, prob, steps):
+    st = prob.copy_state()
+    for _ in range(steps):
+        for s in range(5):
+            oracle.stage(prob, st, P.RK4A[s], P.RK4B[s], prob.dt)
+    return st
+
+
+@pytest.mark.parametrize("n,curved,steps", [(16, False, 100), (32, True, 10)],
+                         ids=["affine K=24576 100 steps", "curved K=196608 10 steps"])
+def test_full_size_trajectory_against_oracle(oracle, n, curved, steps):
+    """The Taylor-Green workloads run for their step counts on the headline
+    path (fused stage, graph-replayed steps) against the oracle run through the
+    same steps (the reference's rk_run loop, euler.c:3691-3699): the state Q,
+    the projected state and the face traces agree to 1e-12 relative L2."""
+    from scratch_wadge_amd.device import DeviceSolver
+    prob = P.tgv3d_problem(n=n, N=3, curved=curved)
+    print(f"case trajectory n={n} curved={curved} K={prob.K} steps={steps}", flush=True)
+    dev = DeviceSolver(prob)
+    dev.run(steps, fused=True, graph=True)
+    g = dev.get_state()          # Qf: the current face-trace buffer
+    del dev
+    torch.cuda.empty_cache()
+    ref = _oracle_steps(oracle, prob, steps)
+    errs = {f: relerr(g[f], ref[f]) for f in ("Q", "rhsQ", "Qf")}
+    print("rel L2 vs oracle: " + " ".join(f"{f} {e:.2e}" for f, e in errs.items()), flush=True)
+    assert max(errs.values()) <= 1e-12
