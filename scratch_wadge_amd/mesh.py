@@ -133,7 +133,12 @@ def tg_warp(x, y, z, a=0.1):
 
 
 def _nodes(dim, N):
-    return refelem.equi_nodes_2d(N) if dim == 2 else refelem.equi_nodes_3d(N)
+    """The degree-N nodes the geometry is interpolated on: the reference's
+    (Nodes2D / Nodes3D, operators.cpp:1780-1840), as the reference maps each
+    element (operators.cpp:357-370, :485-499) and builds the degree-(N+1)
+    Kopriva geofacs (operators.cpp:619-636)."""
+    from .operators import ref_nodes
+    return ref_nodes(dim, N)
 
 
 def _vdm(dim, N, pts):

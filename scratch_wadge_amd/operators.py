@@ -14,11 +14,13 @@ the reference's:
 The polynomial space is spanned with the orthonormal Dubiner basis directly
 (the reference goes through a nodal basis first); every operator above maps
 quadrature values to quadrature values, so the choice of basis only changes
-round-off.  Quadrature rules are this package's own rules with the reference's
-point counts (see quadrature_gen.py): its triangle rules of degree 2-8 are the
-reference's rules, its tetrahedral rules (and the degree-10 triangle rule) are
-other rules of the same size and degree.  use_node_data() takes the rules from a
-reference node_data.h instead.
+round-off.  The quadrature rules are the reference's tabulated ones (the
+tables of its node_data.h that tet_cubature / tri_cubature load,
+operators.cpp:1870-2100: the same points, weights, order and digits), stored
+in quadrature_rules.json, so that volume points, face points and operators are
+the reference's.  The same file keeps this package's own generated rules of the
+same sizes under "<key>_gen" (quadrature_gen.py); use_node_data() reads the
+rules of another node_data.h.
 """
 
 from __future__ import annotations
@@ -50,7 +52,10 @@ def parse_node_data(path):
     tet_cubature / tri_cubature load, operators.cpp:1870-2100): for each degree M
     present, 'tet{M}' from p_rq_N{M}, p_sq_N{M}, p_tq_N{M}, p_wq_N{M} and 'tri{M}'
     from p_rfq_N{M}, p_sfq_N{M}, p_wfq_N{M}, on the biunit reference elements
-    (the same as this package's).  Returns {key: {"points": [[..]], "weights": [..]}}."""
+    (the same as this package's); and for each degree N the tetrahedron's
+    interpolation nodes 'nodes{N}' from p_r_N{N}, p_s_N{N}, p_t_N{N} (Nodes3D,
+    operators.cpp:1791-1840).  Returns {key: {"points": [[..]], "weights": [..]}}
+    (no weights for the node sets)."""
     import re
     with open(path) as f:
         src = f.read()
@@ -59,6 +64,11 @@ def parse_node_data(path):
         arrays[m.group(1)] = [float(x) for x in m.group(2).split(",") if x.strip()]
     rules = {}
     for name in arrays:
+        m = re.fullmatch(r"p_r_N(\d+)", name)
+        if m and all(f"p_{c}_N{m.group(1)}" in arrays for c in "st"):
+            n = m.group(1)
+            pts = np.stack([arrays[f"p_{c}_N{n}"] for c in "rst"], 1)
+            rules[f"nodes{n}"] = {"points": pts.tolist()}
         m = re.fullmatch(r"p_wq_N(\d+)", name)
         if m and all(f"p_{c}q_N{m.group(1)}" in arrays for c in "rst"):
             M = m.group(1)
@@ -95,6 +105,20 @@ def tri_rule(degree):
     r = _rule(f"tri{degree}")
     x = np.array(r["points"])
     return x[:, 0], x[:, 1], np.array(r["weights"])
+
+
+def ref_nodes(dim, N):
+    """Interpolation nodes of degree N on the reference element, the
+    reference's tabulated Nodes3D set (operators.cpp:1791-1840); in 2D its
+    first (N+1)(N+2)/2 nodes, which lie on the face t = -1 (Nodes2D,
+    operators.cpp:1780-1789).  Returns (r, s[, t])."""
+    x = np.array(_rule(f"nodes{N}")["points"])
+    if dim == 2:
+        npt = (N + 1) * (N + 2) // 2
+        if not np.all(x[:npt, 2] == -1.0):
+            raise ValueError(f"the first {npt} nodes of degree {N} are not on the face t = -1")
+        return x[:npt, 0], x[:npt, 1]
+    return x[:, 0], x[:, 1], x[:, 2]
 
 
 def tet_rule(degree):

@@ -138,6 +138,45 @@ def vortex_2d(x, y, t=0.0, x0=0.0, y0=0.0, beta=5.0, u0=1.0, v0=1.0, gamma=GAMMA
     return np.stack([rho, rho * u, rho * v, E], 0)
 
 
+def vortex_3d(x, y, z, t=0.0, gamma=GAMMA):
+    """The reference's 3D isentropic vortex on [0,10] x [0,20] x [0,10]
+    (euler_vortex, euler.c:3324-3351): the 2D vortex extruded along z, centred
+    at (5, 5), moving with unit speed along y; Pi_max = 0.4, p0 = 1/gamma.  The
+    energy is written as in euler.c:3345-3346, p0/(gamma-1) (1 + tmp^gamma) +
+    |m|^2/(2 rho): the pressure p0 (1 + tmp^gamma) is the vortex's p0 tmp^gamma
+    plus a constant p0, which leaves it an exact solution (a uniform pressure
+    changes no force)."""
+    gm1 = gamma - 1.0
+    rx = -(y - 5.0 - t)
+    ry = x - 5.0
+    r2 = rx * rx + ry * ry
+    p0 = 1.0 / gamma
+    L = 0.4 * np.exp(0.5 * (1.0 - r2))
+    tmp = (1.0 - 0.5 * gm1 * L * L) ** (1.0 / gm1)
+    rho = tmp
+    u = rx * L
+    v = 1.0 + ry * L
+    w = 0.0 * x
+    E = p0 / gm1 * (1.0 + tmp ** gamma) + 0.5 * rho * (u * u + v * v + w * w)
+    return np.stack([rho, rho * u, rho * v, rho * w, E], 0)
+
+
+def vortex_warp_2d(x, y, a=0.5):
+    """Curvilinear warp of [0,20] x [-5,5] (operators.cpp:374-383)."""
+    xx, yy = x / 20.0, (y + 5.0) / 10.0
+    return (x + 2.0 * a * np.sin(np.pi * xx) * np.sin(2.0 * np.pi * yy),
+            y - a * np.sin(2.0 * np.pi * xx) * np.sin(np.pi * yy))
+
+
+def vortex_warp_3d(x, y, z, a=0.5):
+    """Curvilinear warp of [0,10] x [0,20] x [0,10] (operators.cpp:504-515)."""
+    xx, yy, zz = x / 10.0, y / 20.0, z / 10.0
+    sa = np.sin(np.pi * xx) * np.sin(2.0 * np.pi * yy) * np.sin(np.pi * zz)
+    return (x + a * sa,
+            y - 2.0 * a * np.sin(2.0 * np.pi * xx) * np.sin(np.pi * yy) * np.sin(2.0 * np.pi * zz),
+            z + a * sa)
+
+
 def taylor_green(x, y, z, gamma=GAMMA):
     """Taylor-Green vortex, Mach 0.1 (euler.c:3384-3411)."""
     u = np.sin(x) * np.cos(y) * np.cos(z)
@@ -235,6 +274,40 @@ def vortex2d_problem(n=16, N=3, perturb=0.05, seed=1, beta=5.0, x0=0.0, y0=0.0, 
     m = meshmod.periodic_box_tri(n, -5.0, 5.0, perturb=perturb, seed=seed)
     ic = lambda x, y: vortex_2d(x, y, 0.0, x0, y0, beta, u0, v0)
     return make_problem(2, N, m, ic, tau=tau, cfl=cfl)
+
+
+def rk_steps(dt, final_time):
+    """Step count and step of rk_run (euler.c:3678-3683): ceil(T/dt) steps of T/steps."""
+    n = int(math.ceil(final_time / dt))
+    return n, final_time / n
+
+
+def vortex_reference_problem(mesh_path, N, curved=False, cfl=0.5, tau=1.0):
+    """The reference's vortex convergence study on one of its gmsh meshes
+    (docs/main.tex:1622-1632, :1766-1795): 2D, the vortex of euler.c:3290-3305
+    (centre (5, 0), beta = 5, unit speed along x) on [0,20] x [-5,5]; 3D, the
+    vortex of euler.c:3324-3351 on [0,10] x [0,20] x [0,10]; curved=True
+    applies the warp of operators.cpp:374-383 / :504-515 (a = 1/2)."""
+    import os
+    dim = 3 if "periodicCube" in os.path.basename(mesh_path) else 2
+    m = meshmod.mesh_from_gmsh(mesh_path, dim)
+    if dim == 2:
+        ic = lambda x, y: vortex_2d(x, y, 0.0, 5.0, 0.0, 5.0, 1.0, 0.0)
+        warp = vortex_warp_2d if curved else None
+    else:
+        ic = lambda x, y, z: vortex_3d(x, y, z, 0.0)
+        warp = vortex_warp_3d if curved else None
+    return make_problem(dim, N, m, ic, warp=warp, tau=tau, cfl=cfl)
+
+
+def vortex_reference_exact(prob, t):
+    """The exact vortex of vortex_reference_problem at time t at the volume
+    quadrature points, in the Q layout."""
+    if prob.dim == 2:
+        x, y = prob.xq
+        return _to_layout(vortex_2d(x, y, t, 5.0, 0.0, 5.0, 1.0, 0.0))
+    x, y, z = prob.xq
+    return _to_layout(vortex_3d(x, y, z, t))
 
 
 def tgv3d_problem(n=16, N=3, curved=False, warp_amp=0.1, tau=1.0, cfl=0.05):

@@ -12,9 +12,13 @@ tri_cubature, operators.cpp:1870, :2023): triangle 3/6/12/16/25 points for
 degree 2/4/6/8/10 and tetrahedron 4/11/23/44 points for degree 2/4/6/8, so a
 degree-N element has the same Nq and Nfq as in the reference.
 
-Run as ``python -m scratch_wadge_amd.quadrature_gen`` to regenerate
-``quadrature_rules.json``; the tests re-check every stored rule against the
-exact monomial integrals.
+Run as ``python -m scratch_wadge_amd.quadrature_gen`` to regenerate the
+``<key>_gen`` entries of ``quadrature_rules.json``; the tests re-check every
+stored rule against the exact monomial integrals.  The operators are built
+from the reference's tabulated rules (the ``<key>`` entries, from its
+node_data.h); the generated ones are kept as an independent check: the
+triangle rules of degree 2-8 are the reference's point sets, the others are
+other rules of the same sizes.
 """
 
 from __future__ import annotations
@@ -267,9 +271,9 @@ EXPECTED_COUNTS = {(2, 2): 3, (2, 4): 6, (2, 6): 12, (2, 8): 16, (2, 10): 25,
 
 def _store(rules, dim, deg, x, w, st):
     order = np.lexsort(tuple(x[:, d] for d in reversed(range(dim))))
-    rules[f"{'tri' if dim == 2 else 'tet'}{deg}"] = {
+    rules[f"{'tri' if dim == 2 else 'tet'}{deg}_gen"] = {
         "degree": deg, "points": x[order].tolist(), "weights": w[order].tolist(),
-        "structure": list(st)}
+        "structure": list(st), "source": "quadrature_gen"}
 
 
 def main(keys=None, out_path=None, tries=20000):

@@ -1,17 +1,19 @@
 """Quadrature rules against the reference's own tables: node_data.h, the file
-tet_cubature / tri_cubature load them from (operators.cpp:1870-2100), read as
-text from a reference checkout when one is present (skipped otherwise; nothing
-of the reference is stored in this repository).
+tet_cubature / tri_cubature load them from (operators.cpp:1870-2100).
 
-* The triangle rules of degree 2, 4, 6 and 8 (the 2D volume rules of N = 1..4
-  and the 3D face rules of N = 1..4) are the reference's: the same points and
-  weights, up to ordering.
-* The tetrahedral rules of degree 4, 6, 8 and the degree-10 triangle rule are
-  other rules of the same size; both they and the reference's integrate every
-  monomial up to their degree exactly.
-* operators.use_node_data() builds the operators from the reference's rules:
-  they pass the hybridized-SBP identities, and the oracle's RHS built on them
-  conserves entropy at tau = 0 on a curved 3D mesh.
+* The package's rule table (quadrature_rules.json) carries the reference's
+  tables for every degree the kernel instances use (triangles of degree 2-10,
+  tetrahedra of degree 2-8): checked against the data fixture
+  tests/golden/node_data_rules.json (written from node_data.h by
+  tests/golden/make_node_data_rules.py), and against the header itself when a
+  reference checkout is present.
+* This package's own generated rules ("<key>_gen", quadrature_gen.py): the
+  triangle rules of degree 2-8 are the reference's point sets; the tetrahedral
+  rules of degree 4, 6, 8 and the degree-10 triangle rule are other rules of
+  the same size; all integrate every monomial up to their degree exactly.
+* operators.use_node_data() builds the operators from a node_data.h: they pass
+  the hybridized-SBP identities, and the oracle's RHS built on them conserves
+  entropy at tau = 0 on a curved 3D mesh.
 """
 
 import itertools
@@ -25,8 +27,40 @@ from scratch_wadge_amd import operators as O
 from test_setup import simplex_monomial
 
 NODE_DATA = os.path.join(os.environ.get("WADGES_REFERENCE", "/root/reference"), "node_data.h")
-pytestmark = pytest.mark.skipif(not os.path.exists(NODE_DATA),
-                                reason="no reference checkout with node_data.h")
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "node_data_rules.json")
+needs_reference = pytest.mark.skipif(not os.path.exists(NODE_DATA),
+                                     reason="no reference checkout with node_data.h")
+KEYS = ["tri2", "tri4", "tri6", "tri8", "tri10", "tet2", "tet4", "tet6", "tet8"]
+
+
+@pytest.mark.parametrize("key", KEYS)
+def test_package_rules_are_the_fixture(key):
+    import json
+    with open(FIXTURE) as f:
+        fix = json.load(f)[key]
+    r = O._rules()[key]
+    assert r["source"] == "node_data.h"
+    assert np.array_equal(np.array(r["points"]), np.array(fix["points"]))
+    assert np.array_equal(np.array(r["weights"]), np.array(fix["weights"]))
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+def test_package_nodes_are_the_fixture(N):
+    """The geometry's interpolation nodes (Nodes3D / Nodes2D) are the reference's
+    tabulated sets; the 2D set is the first (N+1)(N+2)/2 of the 3D set, on the
+    face t = -1, and both interpolate (unisolvent: invertible Vandermonde)."""
+    import json
+    from scratch_wadge_amd import refelem
+    with open(FIXTURE) as f:
+        fix = np.array(json.load(f)[f"nodes{N}"]["points"])
+    assert np.array_equal(np.array(O._rules()[f"nodes{N}"]["points"]), fix)
+    r, s, t = O.ref_nodes(3, N)
+    assert len(r) == (N + 1) * (N + 2) * (N + 3) // 6
+    r2, s2 = O.ref_nodes(2, N)
+    assert np.array_equal(r2, r[:len(r2)]) and np.array_equal(s2, s[:len(s2)])
+    assert np.all(t[:len(r2)] == -1.0)
+    assert np.linalg.cond(refelem.vandermonde_3d(N, r, s, t)) < 1e4
+    assert np.linalg.cond(refelem.vandermonde_2d(N, r2, s2)) < 1e4
 
 
 @pytest.fixture(scope="module")
@@ -59,14 +93,21 @@ def _exact(rule, deg):
     return True
 
 
+@needs_reference
+@pytest.mark.parametrize("key", KEYS)
+def test_fixture_is_the_header(ref_rules, key):
+    assert np.array_equal(np.array(ref_rules[key]["points"]), np.array(O._rules()[key]["points"]))
+    assert np.array_equal(np.array(ref_rules[key]["weights"]), np.array(O._rules()[key]["weights"]))
+
+
 @pytest.mark.parametrize("deg", [2, 4, 6, 8])
-def test_triangle_rules_are_the_references(ref_rules, deg):
-    assert _same_rule(ref_rules[f"tri{deg}"], O._rules()[f"tri{deg}"])
+def test_generated_triangle_rules_are_the_references(deg):
+    assert _same_rule(O._rules()[f"tri{deg}"], O._rules()[f"tri{deg}_gen"])
 
 
 @pytest.mark.parametrize("key", ["tet4", "tet6", "tet8", "tri10"])
-def test_other_rules_same_size_and_degree(ref_rules, key):
-    ours, theirs = O._rules()[key], ref_rules[key]
+def test_generated_rules_same_size_and_degree(key):
+    ours, theirs = O._rules()[f"{key}_gen"], O._rules()[key]
     deg = int(key[3:])
     assert len(ours["weights"]) == len(theirs["weights"])
     assert _exact(ours, deg) and _exact(theirs, deg)
@@ -80,6 +121,7 @@ def reference_rules_in_use():
     O.use_node_data(None)
 
 
+@needs_reference
 def test_operators_from_reference_rules(reference_rules_in_use, oracle):
     assert {"tet6", "tri6"} <= set(reference_rules_in_use)
     ops = O.build_ref_ops(3, 3)
