@@ -227,17 +227,21 @@ typedef struct wadges_group {
 
 typedef struct wadges_batch wadges_batch;
 
-/* Launch shape of the ensemble kernel.  A tile is the floor(64/Nq) elements
- * one 64-wide workgroup holds.  grid = 0: one workgroup per tile (no loop).
- * grid > 0: a persistent launch of grid workgroups over items of
- * tiles_per_item tiles, taken in a static stride (dynamic = 0) or from an
- * atomic work counter (dynamic = 1).  order = 1 issues the costliest items
- * first, 0 keeps problem order. */
+/* Launch shape of the ensemble kernel (the reference's per-kernel
+ * elements-per-workgroup knobs KblkV/S/U, euler.c:342-345, as one object that
+ * the autotuner chooses).  A tile is the elements one 64-wide wave holds
+ * (floor(64/Nq), or floor(64/ceil(Nq/2)) with two rows per lane).  grid = 0:
+ * the flat launch, one tile per wave and waves_per_group (1, 2 or 4) waves per
+ * workgroup, all of one problem (no loop).  grid > 0: a persistent launch of
+ * grid one-wave workgroups over items of tiles_per_item tiles, taken in a
+ * static stride (dynamic = 0) or from an atomic work counter (dynamic = 1).
+ * order = 1 issues the costliest items first, 0 keeps problem order. */
 typedef struct wadges_batch_config {
   int tiles_per_item;
   int grid;
   int dynamic;
   int order;
+  int waves_per_group;
 } wadges_batch_config;
 
 /* Copies the descriptors, checks them (instances exist, pointers set, Qf[0] !=

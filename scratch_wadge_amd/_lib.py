@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "libwadges.so")
 # Every symbol the header declares (checked by the CPU tests).
 EXPORTED = (
     "wadges_plan_create", "wadges_plan_destroy", "wadges_plan_invalidate", "wadges_vol", "wadges_surf", "wadges_update",
-    "wadges_stage", "wadges_stage_record", "wadges_stage_vjp", "wadges_diagnostics", "wadges_rhs_vjp", "wadges_update_vjp",
+    "wadges_stage", "wadges_stage_record", "wadges_stage_vjp", "wadges_diagnostics",
+    "wadges_rhs_vjp", "wadges_update_vjp",
     "wadges_project_vjp", "wadges_batch_create", "wadges_batch_destroy", "wadges_batch_stage",
     "wadges_batch_run", "wadges_batch_current", "wadges_batch_get_config", "wadges_batch_set_config",
     "wadges_batch_tune",
@@ -43,7 +44,7 @@ class Group(ctypes.Structure):
 class BatchConfig(ctypes.Structure):
     """struct wadges_batch_config: launch shape of the ensemble kernel."""
     _fields_ = [("tiles_per_item", ctypes.c_int), ("grid", ctypes.c_int), ("dynamic", ctypes.c_int),
-                ("order", ctypes.c_int)]
+                ("order", ctypes.c_int), ("waves_per_group", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -223,8 +224,8 @@ class Batch:
         check(load().wadges_batch_get_config(self.h, ctypes.byref(c)), "wadges_batch_get_config")
         return c.as_dict()
 
-    def set_config(self, tiles_per_item=1, grid=0, dynamic=0, order=0):
-        c = BatchConfig(tiles_per_item, grid, dynamic, order)
+    def set_config(self, tiles_per_item=1, grid=0, dynamic=0, order=0, waves_per_group=1):
+        c = BatchConfig(tiles_per_item, grid, dynamic, order, waves_per_group)
         check(load().wadges_batch_set_config(self.h, ctypes.byref(c)), "wadges_batch_set_config")
 
     def tune(self, reps=5, stream=None):

@@ -12,7 +12,9 @@ stage, wadges_project_vjp at the end).  Memory follows the usual checkpointing
 scheme: the forward pass keeps the full state every `checkpoint_every` steps;
 the reverse pass re-runs each segment from its checkpoint with
 wadges_stage_record, which writes every stage's state into its own slot (no
-copies), then walks the segment's stages backwards.
+copies), then walks the segment's stages backwards.  The last segment is
+recorded into the slots by the forward pass itself, so the reverse pass
+recomputes all segments but that one.
 
 `RKRollout` wraps the forward map as a torch.autograd.Function whose backward is
 this adjoint.
@@ -50,6 +52,7 @@ class AdjointSolver:
         self.res_work = torch.empty_like(s["Q"])
         self._slots = None
         self._seg_graphs = {}
+        self._recorded = None       # first step of the segment the slots hold
         self.stats = {}
 
     # ------------------------------------------------------------------ forward
@@ -82,16 +85,40 @@ class AdjointSolver:
 
     def forward(self, Q0, nsteps, dt=None):
         """Runs nsteps LSRK45 steps from Q0 (device tensor in the Q layout);
-        keeps checkpoints; returns the final Q (a copy)."""
+        keeps checkpoints; returns the final Q (a copy).  The last segment's
+        stages are recorded into the reverse pass's slots as they are computed
+        (wadges_stage_record): the reverse pass starts with that segment and
+        needs no recomputation for it."""
         dt = self.dev.prob.dt if dt is None else dt
         self.nsteps, self.dt = nsteps, dt
         self.Q0 = Q0.detach().clone()
         self._set_initial(self.Q0)
         self.ckpt = {}
+        self._recorded = None
         for seg in range(0, nsteps, self.C):
             self.ckpt[seg] = self._snapshot()
-            self.dev.run(min(self.C, nsteps - seg), dt, fused=self.fused, graph=self.use_graph)
+            n = min(self.C, nsteps - seg)
+            if seg + self.C >= nsteps and self.fused:
+                self._record(5 * n, dt)
+                self._recorded = seg
+            else:
+                self.dev.run(n, dt, fused=self.fused, graph=self.use_graph)
         return self.dev.s["Q"].clone()
+
+    def _record(self, nst, dt):
+        """nst stages from the solver's state, each stage's state written to its
+        own slot; the solver's state then holds the last one."""
+        S = self._ensure_slots()
+        s = self.dev.s
+        S["Q"][0].copy_(s["Q"])
+        S["rhsQ"][0].copy_(s["rhsQ"])
+        S["Qf"][0].copy_(s["Qf"])
+        self.res_work.copy_(s["resQ"])
+        self._run_segment(nst, dt, reverse=False)
+        s["Q"].copy_(S["Q"][nst])
+        s["rhsQ"].copy_(S["rhsQ"][nst])
+        s["Qf"].copy_(S["Qf"][nst])
+        s["resQ"].copy_(self.res_work)
 
     # ------------------------------------------------------------------ reverse
     def _ensure_slots(self):
@@ -102,8 +129,9 @@ class AdjointSolver:
             self._slots = {"Q": mk(s["Q"]), "rhsQ": mk(s["rhsQ"]), "Qf": mk(s["Qf"])}
         return self._slots
 
-    def _segment(self, nst, dt):
-        """Recompute nst stages from slot 0 into the slots, then walk them back."""
+    def _segment(self, nst, dt, forward=True, reverse=True):
+        """Recompute nst stages from slot 0 into the slots (forward), then walk
+        them back (reverse)."""
         d, c = self.dev, self.dev.c
         plan, K, st = d.plan, d.K, d.stream()
         S = self._slots
@@ -111,28 +139,28 @@ class AdjointSolver:
         geo = (P("vgeo"), P("vfgeo"), P("fgeo"), P("mapPq"), P("nrJ"), P("nsJ"), P("ntJ"), P("Drq"),
                P("Dsq"), P("Dtq"), P("VqLq"), P("VqPq"), P("VfPq"), P("Jq"))
         res = _ptr(self.res_work)
-        for idx in range(nst):
+        for idx in range(nst if forward else 0):
             s = idx % 5
             plan.stage_record(K, *geo, RK4A[s], RK4B[s], dt, _ptr(S["Qf"][idx]), _ptr(S["Qf"][idx + 1]),
                               _ptr(S["rhsQ"][idx]), _ptr(S["rhsQ"][idx + 1]), res, res,
                               _ptr(S["Q"][idx]), _ptr(S["Q"][idx + 1]), st)
         uqb, ufb = _ptr(self.uqb), _ptr(self.ufb)
-        for idx in range(nst - 1, -1, -1):
+        for idx in range(nst - 1 if reverse else -1, -1, -1):
             s = idx % 5
             plan.stage_vjp(K, *geo, RK4A[s], RK4B[s], dt, _ptr(S["rhsQ"][idx]), _ptr(S["Qf"][idx]),
                            _ptr(S["Q"][idx + 1]), _ptr(self.Qb), _ptr(self.resb), uqb, ufb,
                            _ptr(self.Rb), _ptr(self.sbar), uqb, ufb, st)
 
-    def _run_segment(self, nst, dt):
+    def _run_segment(self, nst, dt, forward=True, reverse=True):
         if not self.use_graph:
-            self._segment(nst, dt)
+            self._segment(nst, dt, forward, reverse)
             return
-        key = (nst, dt)
+        key = (nst, dt, forward, reverse)
         g = self._seg_graphs.get(key)
         if g is None:
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._segment(nst, dt)
+                self._segment(nst, dt, forward, reverse)
             self._seg_graphs[key] = g
         g.replay()
 
@@ -145,12 +173,18 @@ class AdjointSolver:
         self.uqb.zero_()
         self.ufb.zero_()
         for seg in sorted(self.ckpt, reverse=True):
+            nst = 5 * (min(seg + self.C, self.nsteps) - seg)
+            if seg == self._recorded:
+                # the forward recorded this segment's stages into the slots
+                self._run_segment(nst, self.dt, forward=False)
+                continue
+            self._recorded = None          # the slots now hold another segment
             ck = self.ckpt[seg]
             S["Q"][0].copy_(ck["Q"])
             S["rhsQ"][0].copy_(ck["rhsQ"])
             S["Qf"][0].copy_(ck["Qf"])
             self.res_work.copy_(ck["resQ"])
-            self._run_segment(5 * (min(seg + self.C, self.nsteps) - seg), self.dt)
+            self._run_segment(nst, self.dt)
         Qb = self.Qb.clone()
         d.plan.project_vjp(d.K, _ptr(c["VqPq"]), _ptr(c["VfPq"]), _ptr(self.Q0), _ptr(self.uqb),
                            _ptr(self.ufb), _ptr(Qb), d.stream())

@@ -301,31 +301,21 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
   if constexpr (ROLLED) vv_steps = opaque_bound(TVT);
   if (valid) {
     // With R > 1 the operator entries of step t + 1 are loaded while step t
-    // computes (one row per lane has no registers to spare for it).
+    // computes (one row per lane has no registers to spare for it).  (Two
+    // buffers used in turn, as in the face walk below, spilled here.)
     constexpr bool PIPE = R > 1;
     const double *Dfw = ws;
     const double *Dbw = ws + DIM * TVT * NQ;
     int iw[R];                        // table row (row 0's for rows that do not exist)
     for (int r = 0; r < R; ++r) iw[r] = rv[r] ? l + r * LPE : 0;
-    double nDf[R][DIM], nDb[R][DIM];
-    auto fetch = [&](int t) {
+    auto fetch = [&](int t, double (&Df)[R][DIM], double (&Db)[R][DIM]) {
       for (int r = 0; r < R; ++r)
         for (int q = 0; q < DIM; ++q) {
-          nDf[r][q] = ld_off(Dfw, (unsigned)((q * TVT + t) * NQ + iw[r]));
-          nDb[r][q] = ld_off(Dbw, (unsigned)((q * TVT + t) * NQ + iw[r]));
+          Df[r][q] = ld_off(Dfw, (unsigned)((q * TVT + t) * NQ + iw[r]));
+          Db[r][q] = ld_off(Dbw, (unsigned)((q * TVT + t) * NQ + iw[r]));
         }
     };
-    if (PIPE) fetch(0);
-    for (int t = 0; t < vv_steps; ++t) {
-      if (2 * t > NQ) break;
-      if (!PIPE) fetch(t);
-      double Df[R][DIM], Db[R][DIM];
-      for (int r = 0; r < R; ++r)
-        for (int q = 0; q < DIM; ++q) {
-          Df[r][q] = nDf[r][q];
-          Db[r][q] = nDb[r][q];
-        }
-      if (PIPE && t + 1 < TVT && 2 * (t + 1) <= NQ) fetch(t + 1);
+    auto vv_step = [&](int t, const double (&Df)[R][DIM], const double (&Db)[R][DIM]) {
       for (int r = 0; r < R; ++r) {
         if (!rv[r]) continue;
         const int i = l + r * LPE;
@@ -335,7 +325,7 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
         const Node<DIM> pj = load_node<DIM>(vrec, NQ, j);
         double Ga[NG];
         for (int c = 0; c < NG; ++c) Ga[c] = hG[r][c] + vrec[(VS + c) * NQ + j];
-        const PairFlux<DIM> P = pair_flux<DIM>(me[r], pj, inv2gm1);
+        const PairFlux<DIM> P = pair_flux<DIM, false, (R > 1)>(me[r], pj, inv2gm1);
         double Dx[DIM];
         for (int d = 0; d < DIM; ++d) Dx[d] = 0.0;
         for (int q = 0; q < DIM; ++q)
@@ -350,6 +340,28 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
           flux_dot<DIM>(P, Dx, c);
           for (int f = 0; f < NFLD; ++f) lds_add(&accV[f * NQ + j], c[f]);
         }
+      }
+    };
+    if constexpr (PIPE) {
+      double nDf[R][DIM], nDb[R][DIM];
+      fetch(0, nDf, nDb);
+      for (int t = 0; t < TVT; ++t) {
+        if (2 * t > NQ) break;
+        double Df[R][DIM], Db[R][DIM];
+        for (int r = 0; r < R; ++r)
+          for (int q = 0; q < DIM; ++q) {
+            Df[r][q] = nDf[r][q];
+            Db[r][q] = nDb[r][q];
+          }
+        if (t + 1 < TVT && 2 * (t + 1) <= NQ) fetch(t + 1, nDf, nDb);
+        vv_step(t, Df, Db);
+      }
+    } else {
+      for (int t = 0; t < vv_steps; ++t) {
+        if (2 * t > NQ) break;
+        double Df[R][DIM], Db[R][DIM];
+        fetch(t, Df, Db);
+        vv_step(t, Df, Db);
       }
     }
   }
@@ -405,29 +417,25 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
           iw[r] = rv[r] ? l + r * LPE : 0;
           if (rv[r]) own_normal(r, h, a[r]);
         }
-        auto fetch = [&](int m, Node<DIM> &pf, double (&b)[DIM], double (&w1)[R], double (&w2)[R]) {
+        // partner record and weights of one step, in two buffers used in turn
+        // (steps taken in pairs: NFQ is even), so no value is copied between steps
+        struct VfStep {
+          Node<DIM> pf;
+          double b[DIM], w1[R], w2[R];
+        };
+        static_assert(NFQ % 2 == 0, "face-walk steps taken in pairs");
+        auto fetch = [&](int m, VfStep &x) {
           int q = l + m;
           if (q >= NFQ) q -= NFQ;
-          pf = load_node<DIM>(frec, NH, q);
-          for (int d = 0; d < DIM; ++d) b[d] = frec[(VS + d) * NH + q];
+          x.pf = load_node<DIM>(frec, NH, q);
+          for (int d = 0; d < DIM; ++d) x.b[d] = frec[(VS + d) * NH + q];
           const int t = h * NFQ + m;
           for (int r = 0; r < R; ++r) {
-            w1[r] = ld_off(W1, (unsigned)(t * NQ + iw[r]));
-            w2[r] = ld_off(W2, (unsigned)(t * NQ + iw[r]));
+            x.w1[r] = ld_off(W1, (unsigned)(t * NQ + iw[r]));
+            x.w2[r] = ld_off(W2, (unsigned)(t * NQ + iw[r]));
           }
         };
-        Node<DIM> nf;
-        double nb[DIM], nw1[R], nw2[R];
-        fetch(0, nf, nb, nw1, nw2);
-        for (int m = 0; m < NFQ; ++m) {
-          const Node<DIM> pf = nf;
-          double b[DIM], w1[R], w2[R];
-          for (int d = 0; d < DIM; ++d) b[d] = nb[d];
-          for (int r = 0; r < R; ++r) {
-            w1[r] = nw1[r];
-            w2[r] = nw2[r];
-          }
-          if (m + 1 < NFQ) fetch(m + 1, nf, nb, nw1, nw2);
+        auto vf_step = [&](int m, const VfStep &x) {
           int q = l + m;
           if (q >= NFQ) q -= NFQ;
           double sF[NFLD];
@@ -435,16 +443,24 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
           for (int r = 0; r < R; ++r) {
             if (!rv[r]) continue;
             double n[DIM];
-            for (int d = 0; d < DIM; ++d) n[d] = a[r][d] + b[d];
-            const PairFlux<DIM> P = pair_flux<DIM>(me[r], pf, inv2gm1);
+            for (int d = 0; d < DIM; ++d) n[d] = a[r][d] + x.b[d];
+            const PairFlux<DIM> P = pair_flux<DIM, false, (R > 1)>(me[r], x.pf, inv2gm1);
             double Fn[NFLD];
             flux_dot<DIM>(P, n, Fn);
             for (int f = 0; f < NFLD; ++f) {
-              acc[r][f] += w1[r] * Fn[f];
-              sF[f] += w2[r] * Fn[f];
+              acc[r][f] += x.w1[r] * Fn[f];
+              sF[f] += x.w2[r] * Fn[f];
             }
           }
           for (int f = 0; f < NFLD; ++f) lds_add(&accF[f * NFN + h * NFQ + q], sF[f]);
+        };
+        VfStep s0, s1;
+        fetch(0, s0);
+        for (int m = 0; m < NFQ; m += 2) {
+          fetch(m + 1, s1);
+          vf_step(m, s0);
+          if (m + 2 < NFQ) fetch(m + 2, s0);
+          vf_step(m + 1, s1);
         }
       } else {
         const int i = l;
@@ -607,7 +623,7 @@ surf_kernel(int K, const double *__restrict__ fgeo, const uint32_t *__restrict__
     const double lamM = sqrt(2.0 * a.hq) + sqrt(gamma * pM / UM[0]);
     const double lamP = sqrt(2.0 * b.hq) + sqrt(gamma * pP / UP[0]);
     const double Lfc = ltau * fmax(lamM, lamP) * sJ;
-    const PairFlux<DIM> P = pair_flux<DIM, DIM == 3>(a, b, inv2gm1);
+    const PairFlux<DIM> P = pair_flux<DIM, DIM == 3, (S::R > 1)>(a, b, inv2gm1);
     double Fn[NFLD];
     flux_dot<DIM>(P, nJ, Fn);
     const double *rf = rhsQf + (size_t)k * NFN * NFLD;
@@ -897,7 +913,7 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
     const double lamM = sqrt(2.0 * a.hq) + sqrt(gamma * pM / UM[0]);
     const double lamP = sqrt(2.0 * b.hq) + sqrt(gamma * pP / UP[0]);
     const double Lfc = ltau * fmax(lamM, lamP) * sJ;
-    const PairFlux<DIM> P = pair_flux<DIM, DIM == 3>(a, b, inv2gm1);
+    const PairFlux<DIM> P = pair_flux<DIM, DIM == 3, (S::R > 1)>(a, b, inv2gm1);
     double Fn[NFLD];
     flux_dot<DIM>(P, nJ, Fn);
     for (int f = 0; f < NFLD; ++f)

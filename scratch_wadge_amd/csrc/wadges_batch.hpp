@@ -104,22 +104,49 @@ batch_stage_persistent(const BatchGroup *__restrict__ groups, const int2 *__rest
   }
 }
 
-// One workgroup per item of one tile: no loop, the register use of the
-// single-problem stage kernels (the largest over the family).  2D: four waves
-// per SIMD declared (the 2D family takes 132 registers unconstrained, which
-// leaves three); 3D: no constraint.
+// One tile of a workgroup of the flat launch: stage_body for every wave, also
+// for a padding tile past the problem's last element (all lanes idle), so that
+// all waves of a workgroup reach the same barriers.
+template <class S>
+__device__ __forceinline__ void flat_tile(const ConstArgs *A, double fa, double fb,
+                                          const double *Qf_in, double *Qf_out, int kfirst,
+                                          int lane, double (*s_nref)[S::DIM], double *smem) {
+  int l;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(l) : "v"(lane));
+  stage_body<S>(A, fa, fb, Qf_in, Qf_out, kfirst, l, s_nref, smem);
+}
+
+template <int DIM, class... Ss>
+__device__ __forceinline__ void flat_dispatch(ShapeList<Ss...>, int slot, const ConstArgs *A,
+                                              double fa, double fb, const double *Qf_in,
+                                              double *Qf_out, int k0, int lane,
+                                              double (*s_nref)[DIM], double *smem) {
+  int idx = 0;
+  ((slot == idx++ ? flat_tile<Ss>(A, fa, fb, Qf_in, Qf_out, k0, lane, s_nref, smem) : void()),
+   ...);
+}
+
+// One tile per wave, W waves per workgroup, no loop: the register use of the
+// single-problem stage kernels (the largest over the family).  The W tiles of
+// a workgroup belong to one problem (the item list pads each problem to a
+// multiple of W tiles), so its waves run the same shape and meet the same
+// barriers; each wave has its own LDS region of smem_wave doubles.  2D: four
+// waves per SIMD declared (the 2D family takes 132 registers unconstrained,
+// which leaves three); 3D: no constraint.
 template <int DIM> constexpr int flat_waves() { return DIM == 2 ? 4 : 1; }
 
-template <int DIM, class Fam>
-__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(flat_waves<DIM>())))
+template <int DIM, class Fam, int W>
+__global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(flat_waves<DIM>())))
 batch_stage_flat(const BatchGroup *__restrict__ groups, const int2 *__restrict__ items,
-                 double fa, double fb, int cur) {
+                 double fa, double fb, int cur, int smem_wave) {
   extern __shared__ double smem[];
-  __shared__ double s_nref[DIM + 1][DIM];
-  const int2 w = items[blockIdx.x];
+  __shared__ double s_nref[W][DIM + 1][DIM];
+  const int wave = W > 1 ? __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE) : 0;
+  const int lane = W > 1 ? (int)threadIdx.x - wave * WAVE : (int)threadIdx.x;
+  const int2 w = items[blockIdx.x * W + wave];
   const ConstGroup *G = (const ConstGroup *)groups + w.x;
-  batch_dispatch<DIM>(Fam{}, G->slot, &G->a, fa, fb, G->qf[cur], G->qf[cur ^ 1], w.y, 1, s_nref,
-                      smem);
+  flat_dispatch<DIM>(Fam{}, G->slot, &G->a, fa, fb, G->qf[cur], G->qf[cur ^ 1], w.y, lane,
+                     s_nref[wave], smem + (size_t)wave * smem_wave);
 }
 
 // LSRK45 coefficients (Carpenter & Kennedy 1994), as tabulated in euler.c:2779-2790.
@@ -147,7 +174,7 @@ struct wadges_batch {
   int2 *d_items = nullptr;
   int nitems = 0, items_cap = 0;
   size_t smem = 0;                       // dynamic LDS bytes per workgroup
-  wadges_batch_config cfg{1, 0, 0, 0};
+  wadges_batch_config cfg{1, 0, 0, 0, 1};
   int cur = 0;
   hipStream_t cap = nullptr;
   hipGraph_t graph[2] = {nullptr, nullptr};
@@ -171,28 +198,43 @@ inline void batch_drop_graphs(wadges_batch *b) {
 }
 
 // Item list of a launch shape: (problem, first element) per item, in problem
-// order or costliest first (stable).  Uploaded synchronously: launch shapes
-// change outside the stepping loop.
+// order or costliest first (stable).  The flat launch takes items in
+// workgroups of waves_per_group tiles of one problem, each problem padded with
+// tiles past its last element to a multiple of that.  Uploaded synchronously:
+// launch shapes change outside the stepping loop.
 inline int batch_build_items(wadges_batch *b, wadges_batch_config c) {
   if (c.grid == 0) c.tiles_per_item = 1;   // the flat kernel takes one tile per item
   if (c.grid == 0) c.dynamic = 0;
-  std::vector<int2> items;
-  std::vector<double> icost;
+  if (c.grid > 0) c.waves_per_group = 1;   // the persistent kernel runs one-wave workgroups
+  const int W = c.waves_per_group;
+  std::vector<std::vector<int2>> units;    // items kept together (a flat workgroup's tiles)
+  std::vector<double> ucost;
   for (int q = 0; q < b->ngroups; ++q) {
     const int per = c.tiles_per_item * b->tpw[q];
+    std::vector<int2> unit;
+    double cost = 0.0;
     for (int k0 = 0; k0 < b->g[q].K; k0 += per) {
-      items.push_back(make_int2(q, k0));
-      icost.push_back(b->cost[q] * std::min(per, b->g[q].K - k0));
+      unit.push_back(make_int2(q, k0));
+      cost += b->cost[q] * std::min(per, b->g[q].K - k0);
+      if ((int)unit.size() == W) {
+        units.push_back(unit);
+        ucost.push_back(cost);
+        unit.clear();
+        cost = 0.0;
+      }
+    }
+    if (!unit.empty()) {
+      while ((int)unit.size() < W) unit.push_back(make_int2(q, b->g[q].K));   // idle tiles
+      units.push_back(unit);
+      ucost.push_back(cost);
     }
   }
-  if (c.order == 1) {
-    std::vector<size_t> idx(items.size());
-    for (size_t n = 0; n < idx.size(); ++n) idx[n] = n;
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return icost[x] > icost[y]; });
-    std::vector<int2> sorted(items.size());
-    for (size_t n = 0; n < idx.size(); ++n) sorted[n] = items[idx[n]];
-    items.swap(sorted);
-  }
+  std::vector<size_t> idx(units.size());
+  for (size_t n = 0; n < idx.size(); ++n) idx[n] = n;
+  if (c.order == 1)
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return ucost[x] > ucost[y]; });
+  std::vector<int2> items;
+  for (size_t n : idx) items.insert(items.end(), units[n].begin(), units[n].end());
   const int n = (int)items.size();
   if (n > b->items_cap) {
     if (b->d_items) {
@@ -219,16 +261,31 @@ inline int batch_build_items(wadges_batch *b, wadges_batch_config c) {
   return WADGES_OK;
 }
 
+template <int DIM, class Fam>
+inline void launch_flat(const wadges_batch *b, int W, double fa, double fb, int cur,
+                        hipStream_t st) {
+  const int blocks = b->nitems / W;
+  const int sw = (int)(b->smem / sizeof(double));
+  const size_t lds = b->smem * (size_t)W;
+  if (W == 4)
+    batch_stage_flat<DIM, Fam, 4><<<blocks, WAVE * 4, lds, st>>>(b->d_groups, b->d_items, fa, fb,
+                                                                cur, sw);
+  else if (W == 2)
+    batch_stage_flat<DIM, Fam, 2><<<blocks, WAVE * 2, lds, st>>>(b->d_groups, b->d_items, fa, fb,
+                                                                cur, sw);
+  else
+    batch_stage_flat<DIM, Fam, 1><<<blocks, WAVE, lds, st>>>(b->d_groups, b->d_items, fa, fb, cur,
+                                                            sw);
+}
+
 inline int batch_launch(wadges_batch *b, double fa, double fb, int cur, hipStream_t st) {
   if (b->nitems == 0) return WADGES_OK;
   const int tiles = b->cfg.tiles_per_item;
   if (b->cfg.grid == 0) {
     if (b->dim == 2)
-      batch_stage_flat<2, Family2><<<b->nitems, WAVE, b->smem, st>>>(b->d_groups, b->d_items, fa,
-                                                                      fb, cur);
+      launch_flat<2, Family2>(b, b->cfg.waves_per_group, fa, fb, cur, st);
     else
-      batch_stage_flat<3, Family3><<<b->nitems, WAVE, b->smem, st>>>(b->d_groups, b->d_items, fa,
-                                                                      fb, cur);
+      launch_flat<3, Family3>(b, b->cfg.waves_per_group, fa, fb, cur, st);
     return hip_check(hipGetLastError());
   }
   const int grid = std::min(b->cfg.grid, b->nitems);
@@ -269,7 +326,7 @@ inline bool tune_file_lookup(const std::string &sig, wadges_batch_config *c) {
   std::string key;
   wadges_batch_config t;
   bool found = false;
-  while (f >> key >> t.tiles_per_item >> t.grid >> t.dynamic >> t.order)
+  while (f >> key >> t.tiles_per_item >> t.grid >> t.dynamic >> t.order >> t.waves_per_group)
     if (key == sig) {
       *c = t;
       found = true;
@@ -282,7 +339,7 @@ inline void tune_file_store(const std::string &sig, const wadges_batch_config &c
   if (!path || !*path) return;
   std::ofstream f(path, std::ios::app);
   f << sig << ' ' << c.tiles_per_item << ' ' << c.grid << ' ' << c.dynamic << ' ' << c.order
-    << '\n';
+    << ' ' << c.waves_per_group << '\n';
 }
 
 }  // namespace wadges
@@ -457,8 +514,11 @@ int wadges_batch_set_config(wadges_batch *b, const wadges_batch_config *cfg) {
   using namespace wadges;
   if (!b || !cfg) return WADGES_EINVAL;
   if (cfg->tiles_per_item < 1 || cfg->tiles_per_item > 64 || cfg->grid < 0 ||
-      (cfg->dynamic != 0 && cfg->dynamic != 1) || (cfg->order != 0 && cfg->order != 1))
+      (cfg->dynamic != 0 && cfg->dynamic != 1) || (cfg->order != 0 && cfg->order != 1) ||
+      (cfg->waves_per_group != 1 && cfg->waves_per_group != 2 && cfg->waves_per_group != 4))
     return WADGES_EINVAL;
+  // the flat launch's LDS is waves_per_group tiles' worth
+  if (cfg->grid == 0 && b->smem * (size_t)cfg->waves_per_group > 65536) return WADGES_EINVAL;
   int rc = hip_check(hipDeviceSynchronize());  // the item list may be in use
   if (rc) return rc;
   return batch_build_items(b, *cfg);
@@ -520,15 +580,19 @@ int wadges_batch_tune(wadges_batch *b, int reps, double *ms, void *stream) {
     rc = hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &occ, batch_persistent_ptr(b->dim), WAVE, b->smem));
   const int resident = std::max(1, occ) * std::max(1, b->cus);
-  // candidates: the flat launch (one workgroup per tile) in both orders, and
-  // persistent launches of one or two resident waves' worth of workgroups per
-  // CU, static or dynamic, over items of 1, 2 or 4 tiles, in both orders
+  // candidates: the flat launch (one tile per wave) with workgroups of 1, 2 or 4
+  // waves (as LDS allows) in both orders, and persistent launches of one or two
+  // resident waves' worth of one-wave workgroups per CU, static or dynamic,
+  // over items of 1, 2 or 4 tiles, in both orders
   std::vector<wadges_batch_config> cand;
-  for (int order = 0; order < 2; ++order) cand.push_back({1, 0, 0, order});
+  for (int W : {1, 2, 4})
+    if (W == 1 || b->smem * (size_t)W <= 65536)
+      for (int order = 0; order < 2; ++order) cand.push_back({1, 0, 0, order, W});
   for (int tiles : {1, 2, 4})
     for (int m : {1, 2})
       for (int dyn = 0; dyn < 2; ++dyn)
-        for (int order = 0; order < 2; ++order) cand.push_back({tiles, m * resident, dyn, order});
+        for (int order = 0; order < 2; ++order)
+          cand.push_back({tiles, m * resident, dyn, order, 1});
   if (!rc) {
     for (const auto &c : cand) {
         if ((rc = batch_build_items(b, c))) break;

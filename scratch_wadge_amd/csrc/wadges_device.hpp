@@ -12,8 +12,8 @@
 //   * the log branch of the log mean, (aL-aR)/ln(aL/aR), uses the per-node
 //     logs: (aL-aR)/(ln aL - ln aR); the series branch keeps the reference's
 //     thresholds (1e-4 with 4 terms in 3D, 1e-3 with 5 terms in 2D);
-//   * the three divisions of a flux evaluation (two log means, the pressure
-//     average and rho_log/beta_log) share one reciprocal;
+//   * the divisions of a flux evaluation (two log means, the pressure average
+//     and rho_log/beta_log) share two reciprocals, one on smooth states;
 //   * the flux is never materialised as a DIM x NFLD matrix: callers contract it
 //     with a direction vector n (sum_d n_d F_d) through flux_dot().
 // All arithmetic is FP64.
@@ -103,19 +103,35 @@ template <int DIM> __device__ __forceinline__ double logmean_series2(double u) {
     return 2.0 + u * (2.0 / 3.0 + u * (2.0 / 5.0 + u * (2.0 / 7.0 + u * (2.0 / 9.0))));
 }
 
+// 1 / (2 F(u)) for the 3D series F(u) = 1 + u/3 + u^2/5 + u^3/7, as the
+// series of 1/F to the same order: 1 - u/3 - 4u^2/45 - 44u^3/945 (the
+// coefficients c_n = -sum_k a_k c_{n-k} of the reciprocal of a power series).
+// For u below the series threshold 1e-4 the omitted term is below 1e-17
+// relative, under the rounding of the division it replaces.
+__device__ __forceinline__ double logmean_inv_series2_3d(double u) {
+  return 0.5 + u * (-1.0 / 6.0 + u * (-2.0 / 45.0 + u * (-22.0 / 945.0)));
+}
+
 // True when p holds on every active lane of the wave (a scalar test).
 __device__ __forceinline__ bool wave_all(bool p) {
   return __builtin_amdgcn_ballot_w64(p) == __builtin_amdgcn_read_exec();
 }
 
 // inv2gm1 = 1 / (2 (gamma - 1)).  Lanes of a wave may take different log-mean
-// branches: both are evaluated and selected per lane, except in 3D when every
-// active lane takes the series branch for both log means (smooth states:
-// neighbouring nodes differ by far less than the 1 % that the series threshold
-// allows), where the selection is skipped.  Both ways give the same bits.  (In
-// 2D the kernels aim at four or more waves per SIMD, whose register budget
-// cannot hold both branches' values across the test: it spilled.)
-template <int DIM, bool LAZY_LOGS = false>
+// branches: both are evaluated and selected per lane, with two reciprocals per
+// pair (f of both log means, then the quotients).  In 3D, when every active
+// lane takes the series branch for both log means (smooth states: neighbouring
+// nodes differ by far less than the 1 % that the series threshold allows), a
+// wave-uniform test skips the selection.  With ONE_RCP that path also needs
+// one reciprocal only: 1/(rho_a + rho_b) and 1/(beta_a + beta_b) both come
+// from 1/((rho_a + rho_b)(beta_a + beta_b)), and rho_log = (rho_a + rho_b) /
+// (2 F(u_rho)) uses the series of 1/(2F) instead of a division (equal to
+// round-off); without it the skip gives the same bits as the selection.  The
+// kernels whose register budget has room for the second path's values use
+// ONE_RCP.  (In 2D the kernels aim at four or more waves per SIMD, whose
+// register budget cannot hold both branches' values across the test: it
+// spilled.)
+template <int DIM, bool LAZY_LOGS = false, bool ONE_RCP = false>
 __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Node<DIM> &b,
                                                    double inv2gm1) {
   constexpr double TH = (DIM == 3) ? 1.0e-4 : 1.0e-3;
@@ -126,26 +142,35 @@ __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Nod
   const double fr = dr * sb * q1;
   const double fb = db * sr * q1;
   const double ur = fr * fr, ub = fb * fb;
-  const double polyR = logmean_series2<DIM>(ur);
-  const double polyB = logmean_series2<DIM>(ub);
   const bool serR = ur < TH, serB = ub < TH;
-  // logmean = num / den:  series (aL+aR) / (2 F(u)),  log (aL-aR) / (ln aL - ln aR)
-  double numR = sr, denR = polyR, numB = sb, denB = polyB;
-  if (DIM == 2 || !wave_all(serR && serB)) {
-    const double ldR = LAZY_LOGS ? log(a.rho) - log(b.rho) : a.lrho - b.lrho;
-    const double ldB = LAZY_LOGS ? log(a.beta) - log(b.beta) : a.lbeta - b.lbeta;
-    numR = serR ? sr : dr;
-    denR = serR ? polyR : ldR;
-    numB = serB ? sb : db;
-    denB = serB ? polyB : ldB;
-  }
-  // rholog = numR/denR, rholog/betalog = numR denB/(denR numB), pa = (sr/2)/sb
-  const double q = rcp(denR * numB * sb);
-  const double A = numB * q;
   PairFlux<DIM> P;
-  P.rholog = numR * sb * A;
-  P.pa = 0.5 * sr * denR * A;
-  const double ratio = numR * denB * sb * q;
+  double ratio;   // rho_log / beta_log
+  const bool all_series = DIM == 3 && wave_all(serR && serB);
+  if (ONE_RCP && all_series) {
+    const double isb = sr * q1;                       // 1 / sb
+    P.rholog = sr * logmean_inv_series2_3d(ur);
+    P.pa = 0.5 * sr * isb;
+    ratio = P.rholog * logmean_series2<DIM>(ub) * isb;   // beta_log = sb / (2 F(u_beta))
+  } else {
+    const double polyR = logmean_series2<DIM>(ur);
+    const double polyB = logmean_series2<DIM>(ub);
+    // logmean = num / den:  series (aL+aR) / (2 F(u)),  log (aL-aR) / (ln aL - ln aR)
+    double numR = sr, denR = polyR, numB = sb, denB = polyB;
+    if (!all_series) {
+      const double ldR = LAZY_LOGS ? log(a.rho) - log(b.rho) : a.lrho - b.lrho;
+      const double ldB = LAZY_LOGS ? log(a.beta) - log(b.beta) : a.lbeta - b.lbeta;
+      numR = serR ? sr : dr;
+      denR = serR ? polyR : ldR;
+      numB = serB ? sb : db;
+      denB = serB ? polyB : ldB;
+    }
+    // rholog = numR/denR, rholog/betalog = numR denB/(denR numB), pa = (sr/2)/sb
+    const double q = rcp(denR * numB * sb);
+    const double A = numB * q;
+    P.rholog = numR * sb * A;
+    P.pa = 0.5 * sr * denR * A;
+    ratio = numR * denB * sb * q;
+  }
   double vv = 0.0;
   for (int d = 0; d < DIM; ++d) {
     P.va[d] = a.hv[d] + b.hv[d];
@@ -279,11 +304,48 @@ __device__ __forceinline__ void flux_dot_vjp_k(const Node<DIM> &a, const Node<DI
                                                const double (&c)[NT][DIM + 2], double inv2gm1,
                                                PrimBar<DIM> &ga, PrimBar<DIM> &gb) {
   constexpr int NF = DIM + 2;
-  double rl, rla, rlb, bl, bla, blb;
-  logmean_partials<DIM>(a.rho, b.rho, a.lrho, b.lrho, rl, rla, rlb);
-  logmean_partials<DIM>(a.beta, b.beta, a.lbeta, b.lbeta, bl, bla, blb);
+  double rl, rla, rlb, bl, bla, blb, isb, ibl;
   const double sr = a.rho + b.rho, sb = a.beta + b.beta;
-  const double isb = rcp(sb);
+  constexpr double TH = (DIM == 3) ? 1.0e-4 : 1.0e-3;
+  const double dr = a.rho - b.rho, db = a.beta - b.beta;
+  bool series = false;
+  if constexpr (DIM == 3) {
+    // f of both log means from one reciprocal; when every active lane takes
+    // the series branch of both (smooth states), 1/F comes from the series of
+    // 1/F and every other reciprocal of the pair from that one.
+    const double q = rcp(sr * sb);
+    const double isr = sb * q;
+    const double isb1 = sr * q;
+    const double fr = dr * isr, fb = db * isb1;
+    const double ur = fr * fr, ub = fb * fb;
+    series = wave_all(ur < TH && ub < TH);
+    if (series) {
+      // l = s/(2F(u)); dl/da = 1/(2F) - l' b, dl/db = 1/(2F) + l' a with
+      // l' = 2 f F'(u) / (s F^2)  (the partials of the series branch)
+      auto partials = [&](double sa, double is, double f, double u, double x, double y,
+                          double &l, double &dx, double &dy, double &i2F) {
+        const double h = logmean_inv_series2_3d(u);          // 1/(2F)
+        const double Fp = 1.0 / 3.0 + u * (2.0 / 5.0 + u * (3.0 / 7.0));
+        l = sa * h;
+        const double t = 8.0 * f * Fp * is * h * h;           // 2 f F' / (s F^2)
+        dx = h - t * y;
+        dy = h + t * x;
+        i2F = h;
+      };
+      double hR, hB;
+      partials(sr, isr, fr, ur, a.rho, b.rho, rl, rla, rlb, hR);
+      partials(sb, isb1, fb, ub, a.beta, b.beta, bl, bla, blb, hB);
+      isb = isb1;
+      ibl = isb1 * logmean_series2<DIM>(ub);                  // 1/bl = 2F(u_beta)/sb
+      (void)hR;
+      (void)hB;
+    }
+  }
+  if (!series) {
+    logmean_partials<DIM>(a.rho, b.rho, a.lrho, b.lrho, rl, rla, rlb);
+    logmean_partials<DIM>(a.beta, b.beta, a.lbeta, b.lbeta, bl, bla, blb);
+    isb = rcp(sb);
+  }
   const double pa = 0.5 * sr * isb;
   double va[DIM], vv = 0.0;
   for (int d = 0; d < DIM; ++d) {
@@ -291,7 +353,7 @@ __device__ __forceinline__ void flux_dot_vjp_k(const Node<DIM> &a, const Node<DI
     vv = vv + va[d] * va[d];
   }
   const double vnavg = 2.0 * vv - (a.hq + b.hq);
-  const double ibl = rcp(bl);
+  if (!series) ibl = rcp(bl);
   const double f4 = rl * inv2gm1 * ibl + pa + 0.5 * rl * vnavg;
   double Un[NT], c0m[NT], A = 0.0, B = 0.0, C = 0.0, tr = 0.0;
   for (int k = 0; k < NT; ++k) {

@@ -41,7 +41,8 @@ def compare(ens, ref, label):
     assert worst <= 1e-14, label
 
 
-CONFIGS = [dict(grid=0), dict(grid=0, order=1), dict(grid=3, tiles_per_item=1),
+CONFIGS = [dict(grid=0), dict(grid=0, order=1), dict(grid=0, waves_per_group=2),
+           dict(grid=0, waves_per_group=4, order=1), dict(grid=3, tiles_per_item=1),
            dict(grid=5, tiles_per_item=2, dynamic=1), dict(grid=7, tiles_per_item=4, dynamic=1, order=1),
            dict(grid=2, tiles_per_item=2, order=1), dict(grid=1000, tiles_per_item=1, dynamic=1)]
 
@@ -53,7 +54,7 @@ def test_ensemble_2d_matches_single_problem(cfg):
     print("case ensemble 2d", cfg, [(p.N, p.K) for p in probs], flush=True)
     ens, ref = Ensemble(probs), PerProblem(probs)
     ens.set_config(**cfg)
-    assert {k: ens.config()[k] for k in cfg} == cfg or cfg.get("grid") == 0
+    assert {k: ens.config()[k] for k in cfg} == cfg
     ens.run(3, use_graph=False)
     ref.run(3)
     compare(ens, ref, f"2d {cfg}")
@@ -77,7 +78,8 @@ def test_ensemble_3d_matches_single_problem():
     from scratch_wadge_amd.ensemble import Ensemble, PerProblem
     probs = mixed_3d()
     print("case ensemble 3d", [(p.N, p.K) for p in probs], flush=True)
-    for cfg in (dict(grid=0), dict(grid=3, tiles_per_item=2, dynamic=1, order=1)):
+    for cfg in (dict(grid=0), dict(grid=0, waves_per_group=2),
+                dict(grid=3, tiles_per_item=2, dynamic=1, order=1)):
         ens, ref = Ensemble(probs), PerProblem(probs)
         ens.set_config(**cfg)
         ens.run(2, use_graph=True)

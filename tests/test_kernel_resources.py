@@ -59,7 +59,7 @@ def test_forward_kernels_do_not_spill(remarks):
         if any(k in name for k in SMALL_SCRATCH):
             flat += 1
             assert r.get("ScratchSize", 0) <= 32, (name[:90], r)
-    assert seen >= 2 * 9 and flat == 2   # vol + stage for 9 shapes, flat ensemble for 2 dims
+    assert seen >= 2 * 9 and flat == 6   # vol + stage for 9 shapes; flat ensemble: 2 dims x 3 widths
 
 
 def test_3d_n3_stage_keeps_two_waves(remarks):
