@@ -57,7 +57,8 @@ enum {
   WADGES_EINVAL = -1,       /* bad argument (null pointer, K < 0, size mismatch) */
   WADGES_EUNSUPPORTED = -2, /* (dim, Nq, Nfq) has no compiled kernel instance   */
   WADGES_EHIP = -3,         /* a HIP runtime call failed (wadges_last_hip_error) */
-  WADGES_ENOMEM = -4        /* device allocation at plan creation failed         */
+  WADGES_ENOMEM = -4,       /* device allocation at plan creation failed         */
+  WADGES_ETOOLARGE = -5     /* size beyond a fixed capacity (wadges_diagnostics_noalloc) */
 };
 
 typedef struct wadges_plan wadges_plan;
@@ -201,6 +202,22 @@ int wadges_project_vjp(const wadges_plan *plan, int K, const double *VqPq, const
 int wadges_diagnostics(const wadges_plan *plan, int K, const double *wq, const double *Jq,
                        const double *Q, const double *Qexact, double *aux, double *totals,
                        void *stream);
+
+/* wadges_diagnostics without the aux argument: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-element partials live in a fixed device array of the library,
+ * WADGES_NOALLOC_MAX_K * 4 doubles (8 MiB) per GPU, held until the process
+ * ends; its address is looked up once per device at plan creation.  K above
+ * WADGES_NOALLOC_MAX_K returns WADGES_ETOOLARGE.  Same kernels and summation
+ * order as wadges_diagnostics, so the totals are bit for bit the same.  The
+ * call allocates nothing.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (different streams, or different host threads) race and
+ * must be ordered by the caller. */
+#define WADGES_NOALLOC_MAX_K 262144
+int wadges_diagnostics_noalloc(const wadges_plan *plan, int K, const double *wq,
+                               const double *Jq, const double *Q, const double *Qexact,
+                               double *totals, void *stream);
 
 /* ---- Ensembles: many independent meshes (uneven K and N, one dim) advanced
  * together, one launch per LSRK stage for the whole ensemble.  No counterpart

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libwadges.so")
 EXPORTED = (
     "wadges_plan_create", "wadges_plan_destroy", "wadges_plan_invalidate", "wadges_vol", "wadges_surf", "wadges_update",
     "wadges_stage", "wadges_stage_record", "wadges_stage_vjp", "wadges_diagnostics",
+    "wadges_diagnostics_noalloc",
     "wadges_rhs_vjp", "wadges_update_vjp",
     "wadges_project_vjp", "wadges_batch_create", "wadges_batch_destroy", "wadges_batch_stage",
     "wadges_batch_run", "wadges_batch_current", "wadges_batch_get_config", "wadges_batch_set_config",
@@ -71,6 +72,7 @@ def load():
     L.wadges_stage_vjp.argtypes = ([_vp, ctypes.c_int] + [_vp] * 14 + [ctypes.c_double] * 3 +
                                    [_vp] * 11 + [_vp])
     L.wadges_diagnostics.argtypes = [_vp, ctypes.c_int] + [_vp] * 6 + [_vp]
+    L.wadges_diagnostics_noalloc.argtypes = [_vp, ctypes.c_int] + [_vp] * 5 + [_vp]
     L.wadges_rhs_vjp.argtypes = [_vp, ctypes.c_int] + [_vp] * 18 + [_vp]
     L.wadges_update_vjp.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double] + [_vp] * 6 + [_vp]

@@ -36,6 +36,7 @@
 #include <fstream>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1158,6 +1159,31 @@ inline int hip_check(hipError_t e) {
   return WADGES_OK;
 }
 
+// Per-element partials of wadges_diagnostics_noalloc: a fixed array of the
+// library, one copy per GPU, so that call allocates nothing.  Host code never
+// passes `scratch` itself to a kernel (on the host the name is only a stand-in
+// for the symbol): it passes the device address that hipGetSymbolAddress
+// returns for the current device, looked up once per device.
+static __device__ __align__(256) double scratch[WADGES_NOALLOC_MAX_K * NDIAG];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be the current device.
+inline int scratch_address(int device, double **addr) {
+  if (device < 0 || device >= kMaxDevices) return WADGES_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void *p = nullptr;
+    const int rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (rc) return rc;
+    g_scratch_addr[device] = (double *)p;
+  }
+  *addr = g_scratch_addr[device];
+  return WADGES_OK;
+}
+
 struct Ops {
   int (*vol)(const wadges_plan *, int, const double *, const double *, const double *,
              const double *, const double *, const double *, const double *, const double *,
@@ -1204,6 +1230,9 @@ struct wadges_plan {
   double gamma, tau;
   double *ws;  // device workspace: two sets of operator tables
   int device;
+  // Device address of the library's fixed diagnostics array on this plan's
+  // device (wadges_diagnostics_noalloc).
+  double *scratch_addr;
   wadges::Ops ops;
   // Operator pointers the tables of the calls outside capture (ws[0, WS)) were
   // prepared from (null: not prepared).
@@ -1475,6 +1504,7 @@ const char *wadges_error_string(int code) {
     case WADGES_EUNSUPPORTED: return "no kernel instance for (dim, Nq, Nfq)";
     case WADGES_EHIP: return "HIP runtime error";
     case WADGES_ENOMEM: return "device allocation failed";
+    case WADGES_ETOOLARGE: return "size beyond the fixed capacity of the call";
     default: return "unknown error";
   }
 }
@@ -1521,6 +1551,10 @@ int wadges_plan_create(wadges_plan **plan, int dim, int N, int Nq, int Nfq, doub
     delete p;
     return WADGES_EHIP;
   }
+  // the fixed diagnostics array of this device (looked up once per device; a
+  // failure here is reported by wadges_diagnostics_noalloc, which retries)
+  p->scratch_addr = nullptr;
+  if (scratch_address(p->device, &p->scratch_addr) != WADGES_OK) p->scratch_addr = nullptr;
   hipError_t e = hipMalloc(&p->ws, sizeof(double) * 2 * (size_t)inst->ops.ws_doubles);
   if (e != hipSuccess) {
     g_last_hip = (int)e;
@@ -1584,6 +1618,26 @@ int wadges_diagnostics(const wadges_plan *p, int K, const double *wq, const doub
                        void *stream) {
   if (!p || K < 0 || !aux || !totals) return WADGES_EINVAL;
   if (K > 0 && (!wq || !Jq || !Q)) return WADGES_EINVAL;
+  return p->ops.diag(p, K, wq, Jq, Q, Qexact, aux, totals, (hipStream_t)stream);
+}
+
+int wadges_diagnostics_noalloc(const wadges_plan *p, int K, const double *wq, const double *Jq,
+                               const double *Q, const double *Qexact, double *totals,
+                               void *stream) {
+  if (!p || K < 0 || !totals) return WADGES_EINVAL;
+  if (K > 0 && (!wq || !Jq || !Q)) return WADGES_EINVAL;
+  if (K > WADGES_NOALLOC_MAX_K) return WADGES_ETOOLARGE;
+  double *aux = p->scratch_addr;
+  if (!aux) {
+    int dev = -1;
+    int rc = hip_check(hipGetDevice(&dev));
+    if (rc) return rc;
+    if (dev != p->device) return WADGES_EINVAL;
+    rc = scratch_address(dev, &aux);
+    if (rc) return rc;
+  }
+  // every entry of aux[0, 4K) is written by diag_kernel before the reduction
+  // reads it, so the array needs no clearing between calls
   return p->ops.diag(p, K, wq, Jq, Q, Qexact, aux, totals, (hipStream_t)stream);
 }
 

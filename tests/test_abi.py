@@ -31,11 +31,12 @@ def test_instances_and_error_strings():
     inst = _lib.instances()
     assert (3, 3, 23, 12) in inst and (3, 4, 44, 16) in inst and (2, 3, 12, 4) in inst
     L = _lib.load()
-    for code in (0, -1, -2, -3, -4):
-        assert L.wadges_error_string(code)
+    for code in (0, -1, -2, -3, -4, -5):
+        assert L.wadges_error_string(code) != b"unknown error"
     # argument checks run before any HIP call
     rc = L.wadges_vol(None, 10, *([None] * 15), None)
     assert rc == -1
+    assert L.wadges_diagnostics_noalloc(None, 10, *([None] * 5), None) == -1
     h = ctypes.c_void_p()
     assert L.wadges_plan_create(ctypes.byref(h), 3, 3, 24, 12, 1.4, 1.0) == -2  # no such instance
     assert L.wadges_plan_create(ctypes.byref(h), 3, 3, 23, 12, 0.9, 1.0) == -1  # gamma <= 1
