@@ -213,12 +213,77 @@ def adjoint3d_bench(steps=8, ckpt=4, n=32):
     t_fwd = _sync_time(lambda: out.setdefault("QT", adj.forward(Q0, steps)))
     t_bwd = _sync_time(lambda: out.setdefault("g", adj.backward(torch.ones_like(Q0))))
     g = out["g"]
+    kernels = vjp_kernel_times(dev, adj)
     return {"workload": f"3D curved TGV {n}^3 x 6 (K={prob.K}), N=3, {steps} steps, "
                         f"checkpoints every {ckpt}",
             "forward_ms": round(t_plain * 1e3, 2), "forward_with_checkpoints_ms": round(t_fwd * 1e3, 2),
             "adjoint_sweep_ms": round(t_bwd * 1e3, 2),
             "adjoint_over_forward": round(t_bwd / t_plain, 3),
-            "grad_finite": bool(torch.isfinite(g).all()), "grad_norm": float(torch.linalg.norm(g))}
+            "grad_finite": bool(torch.isfinite(g).all()), "grad_norm": float(torch.linalg.norm(g)),
+            "vjp_kernels": kernels}
+
+
+def vjp_kernel_times(dev, adj, reps=5):
+    """Launch times (HIP events on the launch stream) of the reverse pass's kernels on
+    the states the sweep left in its slots: the recorded forward stage, the update
+    VJP and the RHS VJP; with the executed FP64 vector-ALU flops per launch of the
+    committed rocprofv3 summary (SQ_INSTS_VALU_*_F64) where it covers them, as the
+    VJP kernels' roofline (they have no algorithmic count of their own)."""
+    import numpy as np
+    import torch
+    from scratch_wadge_amd import workmodel as W
+    from scratch_wadge_amd.device import _ptr
+    from scratch_wadge_amd.problem import RK4A, RK4B
+    c, K, st = dev.c, dev.K, torch.cuda.current_stream()
+    S = adj._ensure_slots()
+    P = lambda n: _ptr(c[n])
+    geo = (P("vgeo"), P("vfgeo"), P("fgeo"), P("mapPq"), P("nrJ"), P("nsJ"), P("ntJ"), P("Drq"),
+           P("Dsq"), P("Dtq"), P("VqLq"), P("VqPq"), P("VfPq"), P("Jq"))
+    dt = adj.dt
+    res = torch.empty_like(dev.s["Q"])
+    Qb, rb = torch.empty_like(dev.s["Q"]), torch.empty_like(dev.s["Q"])
+    uqb, ufb = torch.empty_like(dev.s["rhsQ"]), torch.empty_like(dev.s["Qf"])
+    rhsb, sbar = torch.empty_like(dev.s["Q"]), torch.empty_like(dev.s["Qf"])
+    calls = {
+        "stage_record": lambda: dev.plan.stage_record(
+            K, *geo, RK4A[1], RK4B[1], dt, _ptr(S["Qf"][0]), _ptr(S["Qf"][1]), _ptr(S["rhsQ"][0]),
+            _ptr(S["rhsQ"][1]), _ptr(res), _ptr(res), _ptr(S["Q"][0]), _ptr(S["Q"][1]), st.cuda_stream),
+        "update_vjp": lambda: dev.plan.update_vjp(
+            K, P("Jq"), P("VqPq"), P("VfPq"), RK4A[1], RK4B[1], dt, _ptr(S["Q"][1]), _ptr(Qb), _ptr(rb),
+            _ptr(uqb), _ptr(ufb), _ptr(rhsb), st.cuda_stream),
+        "rhs_vjp": lambda: dev.plan.rhs_vjp(
+            K, *geo[:11], P("VfPq"), _ptr(S["rhsQ"][0]), _ptr(S["Qf"][0]), _ptr(rhsb), _ptr(sbar),
+            _ptr(uqb), _ptr(ufb), st.cuda_stream),
+    }
+    for t in (Qb, rb, uqb, ufb):
+        t.fill_(1e-3)
+    out = {}
+    prof = lambda key, kernel: profiled_value(key, kernel, "adjoint_sweep_tgv3d_curved_K196608_N3.md")
+    for name, fn in calls.items():
+        fn()
+        evs = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(st)
+            fn()
+            b.record(st)
+            evs.append((a, b))
+        torch.cuda.synchronize()
+        ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+        entry = {"launch_ms": round(ms, 4)}
+        kern = {"stage_record": "stage_kernel", "update_vjp": "update_vjp_kernel",
+                "rhs_vjp": "rhs_vjp_kernel"}[name]
+        fl = prof("fp64_lane_flops_per_launch", kern)
+        if fl is not None:
+            tf = fl / (ms * 1e-3) / 1e12
+            entry["roofline"] = {"bound": "fp64 valu", "kernel": kern,
+                                 "counter_tflops": round(tf, 3), "peak": W.FP64_PEAK_TFLOPS,
+                                 "frac": round(tf / W.FP64_PEAK_TFLOPS, 4),
+                                 "flops": "executed FP64 vector-ALU lane flops per launch, "
+                                          "64 (ADD+MUL+TRANS) + 128 FMA (SQ_INSTS_VALU_*_F64)"}
+        out[name] = entry
+    return out
 
 
 def adjoint_bench(steps, ckpt=40):
@@ -482,6 +547,9 @@ def main():
         print(f"# ensemble {out['ensemble']}", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_bench(args.adjoint_steps)
         print(f"# adjoint {out['adjoint']}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        out["adjoint3d"] = adjoint3d_bench()
+        print(f"# adjoint3d {out['adjoint3d']}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -292,6 +292,28 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
   }
   __syncthreads();
 
+  // Smooth waves (3D): when every node of the wave's elements lies within a
+  // relative band SMOOTH_BAND of a reference node (element 0's node 0, read from
+  // LDS) in both rho and beta, every pair has |f| <= band/(1 - band) < 0.0099,
+  // f^2 < 1e-4, and takes the series branch of both log means; the loops then
+  // run without the per-pair wave test (the same arithmetic as the tested path
+  // when it finds every pair in the series branch, so the same bits).
+  constexpr bool SMOOTH_LOOPS = DIM == 3;
+  constexpr double SMOOTH_BAND = 0.0098;
+  const double rho_ref = wc.smem[NFLD * NQ];
+  const double beta_ref = wc.smem[NFLD * NQ + (DIM + 1) * NQ];
+  auto in_band = [&](double rho, double beta) {
+    return fabs(rho - rho_ref) <= SMOOTH_BAND * rho_ref &&
+           fabs(beta - beta_ref) <= SMOOTH_BAND * beta_ref;
+  };
+  bool smooth_v = false;
+  if constexpr (SMOOTH_LOOPS) {
+    bool ok = true;
+    for (int r = 0; r < R; ++r)
+      if (rv[r]) ok = ok && in_band(me[r].rho, me[r].beta);
+    smooth_v = wave_all(ok);
+  }
+
   for (int r = 0; r < R; ++r)
     for (int f = 0; f < NFLD; ++f) acc[r][f] = 0.0;
 
@@ -326,7 +348,7 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
         const Node<DIM> pj = load_node<DIM>(vrec, NQ, j);
         double Ga[NG];
         for (int c = 0; c < NG; ++c) Ga[c] = hG[r][c] + vrec[(VS + c) * NQ + j];
-        const PairFlux<DIM> P = pair_flux<DIM, false, (R > 1)>(me[r], pj, inv2gm1);
+        const PairFlux<DIM> P = pair_flux<DIM, false, (R > 1)>(me[r], pj, inv2gm1, smooth_v);
         double Dx[DIM];
         for (int d = 0; d < DIM; ++d) Dx[d] = 0.0;
         for (int q = 0; q < DIM; ++q)
@@ -343,26 +365,28 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
         }
       }
     };
-    if constexpr (PIPE) {
-      double nDf[R][DIM], nDb[R][DIM];
-      fetch(0, nDf, nDb);
-      for (int t = 0; t < TVT; ++t) {
-        if (2 * t > NQ) break;
-        double Df[R][DIM], Db[R][DIM];
-        for (int r = 0; r < R; ++r)
-          for (int q = 0; q < DIM; ++q) {
-            Df[r][q] = nDf[r][q];
-            Db[r][q] = nDb[r][q];
-          }
-        if (t + 1 < TVT && 2 * (t + 1) <= NQ) fetch(t + 1, nDf, nDb);
-        vv_step(t, Df, Db);
-      }
-    } else {
-      for (int t = 0; t < vv_steps; ++t) {
-        if (2 * t > NQ) break;
-        double Df[R][DIM], Db[R][DIM];
-        fetch(t, Df, Db);
-        vv_step(t, Df, Db);
+    {
+      if constexpr (PIPE) {
+        double nDf[R][DIM], nDb[R][DIM];
+        fetch(0, nDf, nDb);
+        for (int t = 0; t < TVT; ++t) {
+          if (2 * t > NQ) break;
+          double Df[R][DIM], Db[R][DIM];
+          for (int r = 0; r < R; ++r)
+            for (int q = 0; q < DIM; ++q) {
+              Df[r][q] = nDf[r][q];
+              Db[r][q] = nDb[r][q];
+            }
+          if (t + 1 < TVT && 2 * (t + 1) <= NQ) fetch(t + 1, nDf, nDb);
+          vv_step(t, Df, Db);
+        }
+      } else {
+        for (int t = 0; t < vv_steps; ++t) {
+          if (2 * t > NQ) break;
+          double Df[R][DIM], Db[R][DIM];
+          fetch(t, Df, Db);
+          vv_step(t, Df, Db);
+        }
       }
     }
   }
@@ -385,6 +409,7 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
     }
   };
   for (int h = 0; h < H; ++h) {
+    bool ok_f = true;
     for (int p = wc.lane; p < TPW * NH; p += WAVE) {
       const int t = p / NH;
       const int fl = p - t * NH;
@@ -396,7 +421,9 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
       const int fn = h * NH + fl;
       double U[NFLD];
       for (int f = 0; f < NFLD; ++f) U[f] = Uft[fn + f * NFN];
-      store_node<DIM>(frect, NH, fl, make_node<DIM>(U, gm1));
+      const Node<DIM> nf = make_node<DIM>(U, gm1);
+      if constexpr (SMOOTH_LOOPS) ok_f = ok_f && in_band(nf.rho, nf.beta);
+      store_node<DIM>(frect, NH, fl, nf);
       const int face = fn / NFQ;
       for (int d = 0; d < DIM; ++d) {
         double b = 0.0;
@@ -405,6 +432,7 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
       }
     }
     __syncthreads();
+    const bool smooth_f = SMOOTH_LOOPS && smooth_v && wave_all(ok_f);
     if (valid) {
       if constexpr (R > 1) {
         // part h is face h; at step m lane l visits its point (l + m) mod NFQ,
@@ -445,7 +473,7 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
             if (!rv[r]) continue;
             double n[DIM];
             for (int d = 0; d < DIM; ++d) n[d] = a[r][d] + x.b[d];
-            const PairFlux<DIM> P = pair_flux<DIM, false, (R > 1)>(me[r], x.pf, inv2gm1);
+            const PairFlux<DIM> P = pair_flux<DIM, false, (R > 1)>(me[r], x.pf, inv2gm1, smooth_f);
             double Fn[NFLD];
             flux_dot<DIM>(P, n, Fn);
             for (int f = 0; f < NFLD; ++f) {
@@ -472,7 +500,7 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
           const Node<DIM> pf = load_node<DIM>(frec, NH, fl);
           double n[DIM];
           for (int d = 0; d < DIM; ++d) n[d] = a[d] + frec[(VS + d) * NH + fl];
-          const PairFlux<DIM> P = pair_flux<DIM>(me[0], pf, inv2gm1);
+          const PairFlux<DIM> P = pair_flux<DIM>(me[0], pf, inv2gm1, smooth_f);
           double Fn[NFLD];
           flux_dot<DIM>(P, n, Fn);
           const double w1 = ld_off(W1, (unsigned)(t * NQ + i));
@@ -490,20 +518,22 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
             fh_steps = opaque_bound(FH);
             m_steps = opaque_bound(NFQ);
           }
-          for (int u = 0; u < fh_steps; ++u) {
-            // face of this group at part-step u; the chunk c of vf_partner that visits it
-            int fh = u + g;
-            while (fh >= FH) fh -= FH;
-            const int face = h * FH + fh;
-            int c = face - g;
-            if (c < 0) c += S::NFACES;
-            double a[DIM];
-            own_normal(0, face, a);
-            for (int m = 0; m < m_steps; ++m) {
-              int node = o + m;
-              if (node >= NFQ) node -= NFQ;
-              const int fn = face * NFQ + node;
-              vf_pair(c * NFQ + m, fn, fn - h * NH, a);
+          {
+            for (int u = 0; u < fh_steps; ++u) {
+              // face of this group at part-step u; the chunk c of vf_partner that visits it
+              int fh = u + g;
+              while (fh >= FH) fh -= FH;
+              const int face = h * FH + fh;
+              int c = face - g;
+              if (c < 0) c += S::NFACES;
+              double a[DIM];
+              own_normal(0, face, a);
+              for (int m = 0; m < m_steps; ++m) {
+                int node = o + m;
+                if (node >= NFQ) node -= NFQ;
+                const int fn = face * NFQ + node;
+                vf_pair(c * NFQ + m, fn, fn - h * NH, a);
+              }
             }
           }
         } else {
@@ -648,6 +678,74 @@ surf_kernel(int K, const double *__restrict__ fgeo, const uint32_t *__restrict__
   }
 }
 
+// ------------------------------------------------------------------ FP64 matrix-core tiles
+//
+// v_mfma_f64_16x16x4f64: D (16 x 16) = A (16 x 4) B (4 x 16) + C.  Lane l of the
+// wave supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds
+// D[l / 16 + 4 r][l % 16] in component r.  All 64 lanes take part.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma_f64_16x16x4(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+
+// One product on the matrix cores for all elements of a wave:
+// OUT[c][i] = sum_j M[i][j] IN[c][j], i < ROWS, j < KD, for the columns
+// c = t NFLD + f (element t < TPW of the wave, field f).  M is column major
+// (M[i + j LDM]), in global memory or, with M_LDS, in LDS; IN and OUT are
+// per-element LDS arrays, element t's field f at smem + t PER + in_off + f LDIN
+// and smem + t PER + out_off + f LDOUT.  The product is padded to 16-row,
+// 16-column and 4-deep tiles with zeros (elements at or beyond nvalid give zero
+// columns, which are not stored).  Each column's sum runs over j in ascending
+// order, one fused multiply-add per term, as the vector-ALU row products do (the
+// results are the same bits).  Every lane of the wave must call this, and IN
+// must not overlap OUT.
+template <int NFLD, int TPW, int PER, int ROWS, int KD, int LDM, int LDIN, int LDOUT,
+          bool M_LDS = false>
+__device__ __forceinline__ void wave_product_mfma(const double *M, int in_off, int out_off,
+                                                  double *smem, int lane, int nvalid) {
+  constexpr int NCOL = TPW * NFLD;
+  constexpr int MT = (ROWS + 15) / 16, NT = (NCOL + 15) / 16, KS = (KD + 3) / 4;
+  const int lr = lane & 15, lk = lane >> 4;
+  dbl4 acc[MT][NT];
+  for (int mt = 0; mt < MT; ++mt)
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = dbl4{0.0, 0.0, 0.0, 0.0};
+  int boff[NT], ooff[NT];
+  bool bok[NT];
+  for (int nt = 0; nt < NT; ++nt) {
+    const int c = 16 * nt + lr;
+    const int t = c / NFLD;
+    const int f = c - t * NFLD;
+    bok[nt] = c < NCOL && t < nvalid;
+    boff[nt] = bok[nt] ? t * PER + in_off + f * LDIN : 0;
+    ooff[nt] = bok[nt] ? t * PER + out_off + f * LDOUT : 0;
+  }
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const int j = 4 * ks + lk;
+    double a[MT], b[NT];
+    for (int mt = 0; mt < MT; ++mt) {
+      const int row = 16 * mt + lr;
+      if (j < KD && row < ROWS)
+        a[mt] = M_LDS ? M[row + j * LDM] : ld_off(M, (unsigned)(row + j * LDM));
+      else
+        a[mt] = 0.0;
+    }
+    for (int nt = 0; nt < NT; ++nt) b[nt] = (j < KD && bok[nt]) ? smem[boff[nt] + j] : 0.0;
+    for (int mt = 0; mt < MT; ++mt)
+      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma_f64_16x16x4(a[mt], b[nt], acc[mt][nt]);
+  }
+  for (int nt = 0; nt < NT; ++nt) {
+    if (!bok[nt]) continue;
+    for (int mt = 0; mt < MT; ++mt)
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * mt + lk + 4 * r;
+        if (i < ROWS) smem[ooff[nt] + i] = acc[mt][nt][r];
+      }
+  }
+}
+
 // ------------------------------------------------------------------ update kernel
 
 // Launch shape of update_kernel: lane l of an element's LPE lanes holds rows
@@ -679,8 +777,9 @@ update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ V
   constexpr bool PF_LDS = UpdShape<S>::PF_LDS;
   __shared__ double sP[NQ * NQ];
   __shared__ double sPf[PF_LDS ? NFN * NQ : 1];
-  __shared__ double bufA[EPB][NFLD][NQ];
-  __shared__ double bufB[EPB][NFLD][NQ];
+  __shared__ double buf[2][EPB][NFLD][NQ];
+  double (*bufA)[NFLD][NQ] = buf[0];
+  double (*bufB)[NFLD][NQ] = buf[1];
   for (int idx = threadIdx.x; idx < NQ * NQ; idx += blockDim.x) sP[idx] = VqPq[idx];
   if constexpr (PF_LDS)
     for (int idx = threadIdx.x; idx < NFN * NQ; idx += blockDim.x) sPf[idx] = VfPq[idx];
@@ -702,19 +801,19 @@ update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ V
     rv[x] = valid && ir[x] < NQ;
     if (!rv[x]) ir[x] = l;
   }
-  // out[x][f] = sum_j c M[row x, j] src[f][j], M = VqPq, j ascending
-  auto mv = [&](double (*src)[NQ], double c, double (&out)[R][NFLD]) {
-    for (int x = 0; x < R; ++x)
-      for (int f = 0; f < NFLD; ++f) out[x][f] = 0.0;
-#pragma unroll 2
-    for (int j = 0; j < NQ; ++j) {
-      double d[NFLD];
-      for (int f = 0; f < NFLD; ++f) d[f] = src[f][j];
-      for (int x = 0; x < R; ++x) {
-        const double P = c * sP[ir[x] + j * NQ];
-        for (int f = 0; f < NFLD; ++f) out[x][f] += P * d[f];
-      }
-    }
+  // out = c VqPq buf[src] for this lane's rows: the wave's elements' product on
+  // the matrix cores (left in buf[1 - src]), read back by rows.  All threads
+  // call it.
+  const int kw = blockIdx.x * EPB + wave * TPW;           // first element of this wave
+  const int nvalid = K - kw < TPW ? (K - kw < 0 ? 0 : K - kw) : TPW;
+  auto product = [&](int src, double c, double (&out)[R][NFLD]) {
+    constexpr int PE = NFLD * NQ, OFF = EPB * NFLD * NQ;
+    wave_product_mfma<NFLD, TPW, PE, NQ, NQ, NQ, NQ, NQ, true>(
+        sP, src * OFF, (1 - src) * OFF, &buf[0][wave * TPW][0][0], wl, nvalid);
+    __syncthreads();
+    if (valid)
+      for (int x = 0; x < R; ++x)
+        for (int f = 0; f < NFLD; ++f) out[x][f] = c * buf[1 - src][e][f][ir[x]];
   };
 
   double J[R], invJ[R];
@@ -728,51 +827,52 @@ update_kernel(int K, const double *__restrict__ Jq, const double *__restrict__ V
   }
   __syncthreads();
   // r = -2 VqPq rhs ; bufB = r / J
-  if (valid) {
+  {
     double r[R][NFLD];
-    mv(bufA[e], -2.0, r);
-    for (int x = 0; x < R; ++x)
-      if (rv[x])
-        for (int f = 0; f < NFLD; ++f)
-          bufB[e][f][ir[x]] = (DIM == 3) ? r[x][f] * invJ[x] : r[x][f] / J[x];
+    product(0, -2.0, r);
+    if (valid)
+      for (int x = 0; x < R; ++x)
+        if (rv[x])
+          for (int f = 0; f < NFLD; ++f)
+            bufB[e][f][ir[x]] = (DIM == 3) ? r[x][f] * invJ[x] : r[x][f] / J[x];
   }
   __syncthreads();
   // r = VqPq bufB ; RK update ; bufA = v(Q) J
-  if (valid) {
+  {
     double r[R][NFLD];
-    mv(bufB[e], 1.0, r);
-    for (int x = 0; x < R; ++x) {
-      if (!rv[x]) continue;
-      double U[NFLD], V[NFLD];
-      for (int f = 0; f < NFLD; ++f) {
-        const size_t id = base + ir[x] + f * NQ;
-        const double res = fa * resQ[id] + fdt * r[x][f];
-        resQ[id] = res;
-        U[f] = Q[id] + fb * res;
-        Q[id] = U[f];
+    product(1, 1.0, r);
+    if (valid)
+      for (int x = 0; x < R; ++x) {
+        if (!rv[x]) continue;
+        double U[NFLD], V[NFLD];
+        for (int f = 0; f < NFLD; ++f) {
+          const size_t id = base + ir[x] + f * NQ;
+          const double res = fa * resQ[id] + fdt * r[x][f];
+          resQ[id] = res;
+          U[f] = Q[id] + fb * res;
+          Q[id] = U[f];
+        }
+        entropy_vars<DIM>(U, gamma, V);
+        for (int f = 0; f < NFLD; ++f) bufA[e][f][ir[x]] = V[f] * J[x];
       }
-      entropy_vars<DIM>(U, gamma, V);
-      for (int f = 0; f < NFLD; ++f) bufA[e][f][ir[x]] = V[f] * J[x];
-    }
   }
   __syncthreads();
   // bufB = VqPq bufA / J
-  if (valid) {
+  {
     double r[R][NFLD];
-    mv(bufA[e], 1.0, r);
-    for (int x = 0; x < R; ++x)
-      if (rv[x])
-        for (int f = 0; f < NFLD; ++f)
-          bufB[e][f][ir[x]] = (DIM == 3) ? r[x][f] * invJ[x] : r[x][f] / J[x];
+    product(0, 1.0, r);
+    if (valid)
+      for (int x = 0; x < R; ++x)
+        if (rv[x])
+          for (int f = 0; f < NFLD; ++f)
+            bufB[e][f][ir[x]] = (DIM == 3) ? r[x][f] * invJ[x] : r[x][f] / J[x];
   }
   __syncthreads();
-  // bufA = VqPq bufB   (entropy variables v~)
-  if (valid) {
-    double r[R][NFLD];
-    mv(bufB[e], 1.0, r);
-    for (int x = 0; x < R; ++x)
-      if (rv[x])
-        for (int f = 0; f < NFLD; ++f) bufA[e][f][ir[x]] = r[x][f];
+  // bufA = VqPq bufB   (entropy variables v~: the product leaves them in bufA)
+  {
+    constexpr int PE = NFLD * NQ, OFF = EPB * NFLD * NQ;
+    wave_product_mfma<NFLD, TPW, PE, NQ, NQ, NQ, NQ, NQ, true>(
+        sP, OFF, 0, &buf[0][wave * TPW][0][0], wl, nvalid);
   }
   __syncthreads();
   if (valid) {
@@ -826,6 +926,20 @@ struct StageArgs {
   double gamma, gm1, inv2gm1, ltau, lgm1;
 };
 
+// Room for the fused stage's [NFLD][NFN] face-trace product behind the
+// update's two [NFLD][NQ] buffers.
+template <class S> constexpr bool traces_mfma() {
+  return VolSmem<S>::per_elem >= 2 * S::NFLD * S::NQ + S::NFLD * S::NFN;
+}
+
+// The fused stage's products: elements at the stage's LDS stride.
+template <class S, int ROWS, int KD, int LDM, int LDIN, int LDOUT>
+__device__ __forceinline__ void stage_product_mfma(const double *M, int in_off, int out_off,
+                                                   double *smem, int lane, int nvalid) {
+  wave_product_mfma<S::NFLD, S::TPW, VolSmem<S>::per_elem, ROWS, KD, LDM, LDIN, LDOUT>(
+      M, in_off, out_off, smem, lane, nvalid);
+}
+
 // Stage arguments are read through the constant address space, where they
 // live (kernel arguments, or descriptor tables written by the host): loads are
 // scalar, can be repeated at each use instead of held in registers, and the
@@ -861,7 +975,6 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
 
   load_nref<S>(lane, A->nrJ, A->nsJ, A->ntJ, s_nref);
   double *accV = smem + (active ? team : 0) * PER;
-  double *accF = accV + VolSmem<S>::accF_off;
   double acc[R][NFLD];
   const WaveCtx wc{lane, kfirst, A->K, Qf_in, A->vfgeo, smem};
   vol_body<S>(valid, l, A->rhsQ_in + base, A->vgeo + kk * NQ * NG, s_nref, A->ws, accV, gm1,
@@ -922,24 +1035,18 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   }
   __syncthreads();
   // ---------------- RHS rows = volume part + lift of the face terms
-  // (rows that do not exist read row 0's operator entries and are never stored)
+  const int nvalid = A->K - kfirst < TPW ? A->K - kfirst : TPW;   // elements of this wave
+  static_assert(VolSmem<S>::accF_off >= 2 * NFLD * NQ, "the lift's result goes behind bufA");
   double rhs[R][NFLD];
-  if (valid) {
-    double val[R][NFLD];
+  // the lift of all the wave's elements as one matrix-core product, into the
+  // face-record region (free by now)
+  stage_product_mfma<S, NQ, NFN, NQ, NFN, NQ>(A->VqLq, VolSmem<S>::accF_off, NFLD * NQ, smem, lane,
+                                             nvalid);
+  __syncthreads();
+  if (valid)
     for (int r = 0; r < R; ++r)
-      for (int f = 0; f < NFLD; ++f) val[r][f] = 0.0;
-#pragma unroll 8
-    for (int j = 0; j < NFN; ++j) {
-      double x[NFLD];
-      for (int f = 0; f < NFLD; ++f) x[f] = accF[f * NFN + j];
-      for (int r = 0; r < R; ++r) {
-        const double L = ld_off(A->VqLq, (unsigned)(ir[r] + j * NQ));
-        for (int f = 0; f < NFLD; ++f) val[r][f] += L * x[f];
-      }
-    }
-    for (int r = 0; r < R; ++r)
-      for (int f = 0; f < NFLD; ++f) rhs[r][f] = (acc[r][f] + accV[f * NQ + ir[r]]) + val[r][f];
-  }
+      for (int f = 0; f < NFLD; ++f)
+        rhs[r][f] = (acc[r][f] + accV[f * NQ + ir[r]]) + accV[NFLD * NQ + f * NQ + ir[r]];
   __syncthreads();
 
   // ---------------- update (same sequence as update_kernel)
@@ -955,90 +1062,68 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
     }
   }
   __syncthreads();
-  // Row i of VqPq serves all four products below: for the tetrahedral shapes with
-  // Nq <= 32 and one row per lane it is held in registers (the volume phase's
-  // registers are free by now; in 2D the register budget of the higher
-  // occupancy has no room), else it is read through L1.
-  constexpr bool ROW_IN_REGS = DIM == 3 && R * NQ <= 32;
-  double Prow[ROW_IN_REGS ? R : 1][ROW_IN_REGS ? NQ : 1];
-  if constexpr (ROW_IN_REGS) {
+  // The four VqPq products: on the matrix cores for all the wave's elements at
+  // once, each written to LDS and read back by rows.  res = scale VqPq (buffer
+  // at in_off) for this lane's rows; the product also stays in the buffer at
+  // out_off.  All lanes call it.
+  auto product = [&](int in_off, int out_off, double scale, double (&res)[R][NFLD]) {
+    stage_product_mfma<S, NQ, NQ, NQ, NQ, NQ>(A->VqPq, in_off, out_off, smem, lane, nvalid);
+    __syncthreads();
     if (valid)
       for (int r = 0; r < R; ++r)
-        for (int j = 0; j < NQ; ++j) Prow[r][j] = ld_off(A->VqPq, (unsigned)(ir[r] + j * NQ));
-  }
-  auto matvec = [&](const double *in, double scale, double (&res)[R][NFLD]) {
-    for (int r = 0; r < R; ++r)
-      for (int f = 0; f < NFLD; ++f) res[r][f] = 0.0;
-    if constexpr (ROW_IN_REGS) {
-#pragma unroll
-      for (int j = 0; j < NQ; ++j) {
-        double x[NFLD];
-        for (int f = 0; f < NFLD; ++f) x[f] = in[f * NQ + j];
-        for (int r = 0; r < R; ++r) {
-          const double P = scale * Prow[r][j];
-          for (int f = 0; f < NFLD; ++f) res[r][f] += P * x[f];
-        }
-      }
-    } else {
-#pragma unroll 8
-      for (int j = 0; j < NQ; ++j) {
-        double x[NFLD];
-        for (int f = 0; f < NFLD; ++f) x[f] = in[f * NQ + j];
-        for (int r = 0; r < R; ++r) {
-          const double P = scale * ld_off(A->VqPq, (unsigned)(ir[r] + j * NQ));
-          for (int f = 0; f < NFLD; ++f) res[r][f] += P * x[f];
-        }
-      }
-    }
+        for (int f = 0; f < NFLD; ++f) res[r][f] = scale * accV[out_off + f * NQ + ir[r]];
   };
-  if (valid) {
+  constexpr int OA = 0, OB = NFLD * NQ;   // bufA, bufB
+  {
     double res[R][NFLD];
-    matvec(bufA, -2.0, res);
-    for (int r = 0; r < R; ++r) {
-      if (!rv[r]) continue;
-      for (int f = 0; f < NFLD; ++f)
-        bufB[f * NQ + ir[r]] = (DIM == 3) ? res[r][f] * invJ[r] : res[r][f] / J[r];
-    }
-  }
-  __syncthreads();
-  if (valid) {
-    double res[R][NFLD];
-    matvec(bufB, 1.0, res);
-    for (int r = 0; r < R; ++r) {
-      if (!rv[r]) continue;
-      double U[NFLD], V[NFLD];
-      for (int f = 0; f < NFLD; ++f) {
-        const size_t id = base + ir[r] + f * NQ;
-        const double rq = fa * (EARLY ? resin[r][f] : A->resQ_in[id]) + fdt * res[r][f];
-        A->resQ[id] = rq;
-        U[f] = (EARLY ? Qin[r][f] : A->Q_in[id]) + fb * rq;
-        A->Q[id] = U[f];
+    product(OA, OB, -2.0, res);
+    if (valid)
+      for (int r = 0; r < R; ++r) {
+        if (!rv[r]) continue;
+        for (int f = 0; f < NFLD; ++f)
+          bufB[f * NQ + ir[r]] = (DIM == 3) ? res[r][f] * invJ[r] : res[r][f] / J[r];
       }
-      entropy_vars<DIM>(U, gamma, V);
-      for (int f = 0; f < NFLD; ++f) bufA[f * NQ + ir[r]] = V[f] * J[r];
-    }
   }
   __syncthreads();
-  if (valid) {
+  {
     double res[R][NFLD];
-    matvec(bufA, 1.0, res);
-    for (int r = 0; r < R; ++r) {
-      if (!rv[r]) continue;
-      for (int f = 0; f < NFLD; ++f)
-        bufB[f * NQ + ir[r]] = (DIM == 3) ? res[r][f] * invJ[r] : res[r][f] / J[r];
-    }
+    product(OB, OA, 1.0, res);
+    if (valid)
+      for (int r = 0; r < R; ++r) {
+        if (!rv[r]) continue;
+        double U[NFLD], V[NFLD];
+        for (int f = 0; f < NFLD; ++f) {
+          const size_t id = base + ir[r] + f * NQ;
+          const double rq = fa * (EARLY ? resin[r][f] : A->resQ_in[id]) + fdt * res[r][f];
+          A->resQ[id] = rq;
+          U[f] = (EARLY ? Qin[r][f] : A->Q_in[id]) + fb * rq;
+          A->Q[id] = U[f];
+        }
+        entropy_vars<DIM>(U, gamma, V);
+        for (int f = 0; f < NFLD; ++f) bufA[f * NQ + ir[r]] = V[f] * J[r];
+      }
   }
   __syncthreads();
-  if (valid) {
+  {
     double res[R][NFLD];
-    matvec(bufB, 1.0, res);
-    for (int r = 0; r < R; ++r) {
-      if (!rv[r]) continue;
-      for (int f = 0; f < NFLD; ++f) bufA[f * NQ + ir[r]] = res[r][f];
-    }
+    product(OA, OB, 1.0, res);
+    if (valid)
+      for (int r = 0; r < R; ++r) {
+        if (!rv[r]) continue;
+        for (int f = 0; f < NFLD; ++f)
+          bufB[f * NQ + ir[r]] = (DIM == 3) ? res[r][f] * invJ[r] : res[r][f] / J[r];
+      }
   }
   __syncthreads();
-  // face points, spread over the whole wave: Qf = u(VfPq v~)
+  // entropy variables v~ into bufA
+  stage_product_mfma<S, NQ, NQ, NQ, NQ, NQ>(A->VqPq, OB, OA, smem, lane, nvalid);
+  __syncthreads();
+  // face points, spread over the whole wave: Qf = u(VfPq v~), with VfPq v~ of
+  // all the wave's elements as one matrix-core product
+  static_assert(traces_mfma<S>(), "the traces' entropy variables go behind bufB");
+  constexpr int WOFF = 2 * NFLD * NQ;
+  stage_product_mfma<S, NFN, NQ, NFN, NQ, NFN>(A->VfPq, 0, WOFF, smem, lane, nvalid);
+  __syncthreads();
   for (int p = lane; p < TPW * NFN; p += WAVE) {
     const int t = p / NFN;
     const int fn = p - t * NFN;
@@ -1046,12 +1131,7 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
     if (kt >= A->K) continue;
     const double *bufAt = smem + t * PER;
     double V[NFLD], U[NFLD];
-    for (int f = 0; f < NFLD; ++f) V[f] = 0.0;
-#pragma unroll 8
-    for (int j = 0; j < NQ; ++j) {
-      const double P = ld_off(A->VfPq, (unsigned)(fn + j * NFN));
-      for (int f = 0; f < NFLD; ++f) V[f] += P * bufAt[f * NQ + j];
-    }
+    for (int f = 0; f < NFLD; ++f) V[f] = bufAt[WOFF + f * NFN + fn];
     conservative_vars<DIM>(V, gamma, lgm1, U);
     double *qf = Qf_out + (size_t)kt * NFN * NFLD;
     for (int f = 0; f < NFLD; ++f) qf[fn + f * NFN] = U[f];

@@ -131,9 +131,10 @@ __device__ __forceinline__ void flat_dispatch(ShapeList<Ss...>, int slot, const 
 // a workgroup belong to one problem (the item list pads each problem to a
 // multiple of W tiles), so its waves run the same shape and meet the same
 // barriers; each wave has its own LDS region of smem_wave doubles.  2D: four
-// waves per SIMD declared (the 2D family takes 132 registers unconstrained,
-// which leaves three); 3D: no constraint.
-template <int DIM> constexpr int flat_waves() { return DIM == 2 ? 4 : 1; }
+// waves per SIMD declared; 3D: two, which the LDS of 3D N = 3 allows at most
+// (unconstrained, the matrix-core accumulators of the stage body went to extra
+// registers and left one wave per SIMD).
+template <int DIM> constexpr int flat_waves() { return DIM == 2 ? 4 : 2; }
 
 template <int DIM, class Fam, int W>
 __global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(flat_waves<DIM>())))

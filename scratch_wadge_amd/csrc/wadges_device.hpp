@@ -131,9 +131,12 @@ __device__ __forceinline__ bool wave_all(bool p) {
 // ONE_RCP.  (In 2D the kernels aim at four or more waves per SIMD, whose
 // register budget cannot hold both branches' values across the test: it
 // spilled.)
+// smooth: a wave-uniform flag by which the caller guarantees (3D) that every
+// active lane's pair takes the series branch of both log means; the per-pair
+// wave test is then skipped (the same branch, so the same bits).
 template <int DIM, bool LAZY_LOGS = false, bool ONE_RCP = false>
 __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Node<DIM> &b,
-                                                   double inv2gm1) {
+                                                   double inv2gm1, bool smooth = false) {
   constexpr double TH = (DIM == 3) ? 1.0e-4 : 1.0e-3;
   const double sr = a.rho + b.rho, dr = a.rho - b.rho;
   const double sb = a.beta + b.beta, db = a.beta - b.beta;
@@ -145,7 +148,7 @@ __device__ __forceinline__ PairFlux<DIM> pair_flux(const Node<DIM> &a, const Nod
   const bool serR = ur < TH, serB = ub < TH;
   PairFlux<DIM> P;
   double ratio;   // rho_log / beta_log
-  const bool all_series = DIM == 3 && wave_all(serR && serB);
+  const bool all_series = DIM == 3 && (smooth || wave_all(serR && serB));
   if (ONE_RCP && all_series) {
     const double isb = sr * q1;                       // 1 / sb
     P.rholog = sr * logmean_inv_series2_3d(ur);
