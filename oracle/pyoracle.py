@@ -51,6 +51,7 @@ def lib():
         L.wo_flop_counts.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.wo_set_threads.argtypes = [ctypes.c_int]
         L.wo_set_logmean_variant.argtypes = [ctypes.c_int]
+        L.wo_set_update_variant.argtypes = [ctypes.c_int]
         L.wo_get_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -74,6 +75,12 @@ def set_threads(n):
 def set_logmean_variant(v):
     """0 = intended log mean; 1 = the reference's as-shipped 3D log branch (tests only)."""
     lib().wo_set_logmean_variant(int(v))
+
+
+def set_update_variant(v):
+    """0 = the weight-adjusted update as written; 1 = its first product without the
+    projection, a slip invisible on affine meshes (tests only)."""
+    lib().wo_set_update_variant(int(v))
 
 
 def logmean(dim, a, b):

@@ -94,6 +94,13 @@ inline FlopCount wmax(FlopCount a, FlopCount b) { g_tally.cmp++; return a.v > b.
 // can plant that mistake in the oracle and show that their checks reject it.
 int g_logmean_variant = 0;
 
+// 0: the weight-adjusted update as written.  1: its first product without the
+// projection, r = (-2 rhs) / J instead of (VqPq (-2 rhs)) / J -- a plausible slip
+// that changes nothing on affine meshes (J constant per element).  Variant 1
+// exists only so that the tests can plant that mistake in the oracle and show
+// that the curved-mesh checks reject it.
+int g_update_variant = 0;
+
 // Log mean (aL - aR)/(ln aL - ln aR).
 template <int DIM, typename T> inline T logmean(T aL, T aR) {
   if constexpr (DIM == 3) {
@@ -483,6 +490,8 @@ void update_element(const Sizes &S, double gamma, const T *Jq_k, const T *VqPq, 
       const T P = T(-2.0) * VqPq[i + j * Nq];
       for (int f = 0; f < NF; ++f) r[f] += P * rhsQ_k[j + f * Nq];
     }
+    if (g_update_variant == 1)
+      for (int f = 0; f < NF; ++f) r[f] = T(-2.0) * rhsQ_k[i + f * Nq];
     if constexpr (DIM == 3) {
       const T invJ = T(1.0) / Jq_k[i];
       for (int f = 0; f < NF; ++f) a[i + f * Nq] = r[f] * invJ;
@@ -576,6 +585,7 @@ void wo_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 
 // Test hook: 0 = intended 3D log mean, 1 = the as-shipped log branch (see logmean).
 void wo_set_logmean_variant(int v) { g_logmean_variant = v; }
+void wo_set_update_variant(int v) { g_update_variant = v; }
 
 int wo_get_max_threads(void) {
 #ifdef _OPENMP

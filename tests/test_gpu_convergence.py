@@ -84,3 +84,35 @@ def test_vortex_errors_match_published(dim, curved, N):
     print(f"  observed rates {np.round(rates, 3)}, published {np.round(prates, 3)}", flush=True)
     for err, ref in zip(errs, pub):
         assert abs(err - ref) <= BAR * ref
+
+
+def test_published_errors_pin_the_penalty_constant():
+    """The published errors are sensitive to the Lax-Friedrichs penalty
+    (okl/Euler3D.okl:3316-3330, tau/4 max(lamM, lamP) sJ): with the constant
+    doubled (tau = 2) or the penalty left out (tau = 0), the error of 2D N = 3 on
+    periodicSquare2.msh misses the published one by far more than the bar the
+    reference's discretization meets, so a wrong penalty constant would fail
+    test_vortex_errors_match_published."""
+    import torch
+    path = os.path.join(GOLD, "periodicSquare2.msh")
+    pub = PUBLISHED[(2, False, 3)][1]
+
+    def err(tau):
+        from scratch_wadge_amd.device import DeviceSolver
+        prob = P.vortex_reference_problem(path, 3, curved=False, cfl=0.5, tau=tau)
+        nsteps, dt = P.rk_steps(prob.dt, 5.0)
+        dev = DeviceSolver(prob)
+        dev.run(nsteps, dt=dt, fused=True, graph=True)
+        e = dev.diagnostics(P.vortex_reference_exact(prob, 5.0))["l2_error"]
+        del dev
+        torch.cuda.empty_cache()
+        return e
+
+    rel = lambda e: abs(e - pub) / pub
+    e1 = err(1.0)
+    print(f"case penalty pin: tau=1 L2 error {e1:.7g} (published {pub:.7g})", flush=True)
+    assert rel(e1) <= BAR
+    for tau in (0.0, 2.0):
+        e = err(tau)
+        print(f"  tau={tau}: L2 error {e:.7g}, relative difference {rel(e):.2e}", flush=True)
+        assert rel(e) > 20 * BAR

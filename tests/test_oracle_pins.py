@@ -262,3 +262,73 @@ def test_update_identities(oracle, dim, N):
     oracle.update(prob, 0.0, 1.0, 1.0, st2["rhsQ"], st2["resQ"], st2["Q"], st2["Qf"])
     np.testing.assert_allclose(st2["rhsQ"], st["rhsQ"], rtol=1e-12)
     np.testing.assert_allclose(st2["Qf"], st["Qf"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("dim,N", [(2, 3), (3, 3)])
+def test_curved_update_projects_before_and_after_the_weight(oracle, dim, N):
+    """The weight-adjusted inverse mass is VqPq diag(1/J) VqPq (okl/Euler3D.okl:
+    2586-2703): on a curved mesh (J varying inside elements) an RHS orthogonal
+    to the polynomial space (VqPq rhs = 0) gives a zero increment, and every
+    increment lies in that space (VqPq r = r).  Dropping the first projection, a
+    slip that changes nothing on affine meshes, is planted in the oracle and
+    rejected."""
+    import os
+    from scratch_wadge_amd import problem as P
+    if dim == 2:   # the reference's curved vortex mesh (warp a = 1/2)
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "periodicSquare1.msh")
+        prob = P.vortex_reference_problem(path, N, curved=True)
+    else:
+        prob = small_problem(dim, N, curved=True)
+    Q, rq, qf = rough_state(prob, seed=21)
+    K, nf, Nq = prob.K, prob.nfields, prob.Nq
+    VqPq = prob.ops.VqPq
+    J = prob.Jq.reshape(K, Nq)
+    assert np.ptp(J, axis=1).max() > 1e-3 * np.abs(J).max()      # J varies inside elements
+    rng = np.random.default_rng(3)
+    g = rng.standard_normal((K, nf, Nq))
+    perp = g - np.einsum("ij,kfj->kfi", VqPq, g)                # VqPq perp = 0
+
+    def incr(r):
+        st = {"Q": Q.copy(), "rhsQ": r.reshape(-1).copy(), "Qf": qf.copy(), "resQ": np.zeros_like(Q)}
+        oracle.update(prob, 0.0, 1.0, 1.0, st["rhsQ"], st["resQ"], st["Q"], st["Qf"])
+        return st["resQ"].reshape(K, nf, Nq)
+
+    scale = np.abs(incr(g)).max()
+    assert np.abs(incr(perp)).max() <= 1e-12 * scale
+    r = incr(g)
+    np.testing.assert_allclose(np.einsum("ij,kfj->kfi", VqPq, r), r, rtol=0, atol=1e-12 * scale)
+    try:
+        oracle.set_update_variant(1)
+        planted = incr(perp)
+    finally:
+        oracle.set_update_variant(0)
+    assert np.abs(planted).max() > 1e-6 * scale
+
+
+def test_penalty_constant_pinned_by_published_errors(oracle):
+    """The Lax-Friedrichs penalty tau/4 max(lamM, lamP) sJ (okl/Euler3D.okl:
+    3316-3330; okl/Euler2D.okl has tau = 1): the oracle's isentropic-vortex L2
+    error on the reference's periodicSquare2.msh at N = 3, T = 5 equals the
+    reference's published 0.05160533 (docs/figs/errors_3d.m:6) to 2e-4, while
+    the penalty doubled (tau = 2, a constant of 1/2) or left out (tau = 0)
+    misses it by more than 20 times that."""
+    import os
+    from scratch_wadge_amd import problem as P
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "periodicSquare2.msh")
+    pub = 0.05160533
+
+    def err(tau):
+        prob = P.vortex_reference_problem(path, 3, curved=False, cfl=0.5, tau=tau)
+        nsteps, dt = P.rk_steps(prob.dt, 5.0)
+        st = prob.copy_state()
+        for _ in range(nsteps):
+            for s in range(5):
+                oracle.stage(prob, st, P.RK4A[s], P.RK4B[s], dt)
+        K, nf, Nq = prob.K, prob.nfields, prob.Nq
+        wJ = prob.wq[None, :] * prob.Jq.reshape(K, Nq)
+        d = (st["Q"] - P.vortex_reference_exact(prob, 5.0)).reshape(K, nf, Nq)
+        return float(np.sqrt(np.sum(wJ[:, None, :] * d * d)))
+
+    assert abs(err(1.0) - pub) <= 2e-4 * pub
+    for tau in (0.0, 2.0):
+        assert abs(err(tau) - pub) > 4e-3 * pub
