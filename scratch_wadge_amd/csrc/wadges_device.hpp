@@ -117,6 +117,27 @@ __device__ __forceinline__ bool wave_all(bool p) {
   return __builtin_amdgcn_ballot_w64(p) == __builtin_amdgcn_read_exec();
 }
 
+// A flag the caller knows to be the same on every lane, as a value the
+// compiler knows to be uniform (a scalar register, tested with a scalar branch
+// instead of lane masks).
+__device__ __forceinline__ bool uniform_flag(bool p) {
+  return __builtin_amdgcn_readfirstlane((int)p) != 0;
+}
+
+// Smooth waves (3D).  A node is in band when its rho and beta lie within a
+// relative kSmoothBand of a reference node's.  When every node of a wave's
+// elements is in band, every pair (a, b) of them has |a - b| / (a + b) <=
+// band / (1 - band) < 0.0099, so f^2 < 1e-4: both log means of every pair take
+// the 3D series branch, and the pair routines may skip their per-pair wave test
+// (the branch they take is the same, so are the bits).
+constexpr double kSmoothBand = 0.0098;
+
+__device__ __forceinline__ bool in_smooth_band(double rho, double beta, double rho_ref,
+                                               double beta_ref) {
+  return fabs(rho - rho_ref) <= kSmoothBand * rho_ref &&
+         fabs(beta - beta_ref) <= kSmoothBand * beta_ref;
+}
+
 // inv2gm1 = 1 / (2 (gamma - 1)).  Lanes of a wave may take different log-mean
 // branches: both are evaluated and selected per lane, with two reciprocals per
 // pair (f of both log means, then the quotients).  In 3D, when every active
