@@ -100,7 +100,11 @@ template <int DIM_, int NQ_, int NFQ_> struct Shape {
   static constexpr int LPE = (NQ + R - 1) / R;
   static constexpr int TPW = WAVE / LPE;
   static constexpr int WS_R = R > 1 ? 2 * NFN * NQ : 0;   // face-walk weights (R > 1)
-  static constexpr int WS = WS_D + WS_W + WS_R;
+  // One element per wave with one face point per lane (3D N = 4: Nq = 44, Nf =
+  // 64): the volume-face block is walked by face point, every lane busy.
+  static constexpr bool COLVF = R == 1 && TPW == 1 && NFN == WAVE && NQ < WAVE;
+  static constexpr int WS_C = COLVF ? 2 * NQ * NFN : 0;   // its weights
+  static constexpr int WS = WS_D + WS_W + WS_R + WS_C;
   static_assert(R == 1 || (NQ <= NFN && LPE <= NFQ), "R > 1 needs one element's lanes on one face");
 };
 
@@ -185,6 +189,15 @@ __global__ void prep_vol_tables(const double *__restrict__ Drq, const double *__
     if (q >= S::NFQ) q -= S::NFQ;
     const int f = (t / S::NFQ) * S::NFQ + q;
     ws[idx] = s == 0 ? 0.5 * VqLq[i + f * NQ] : -0.5 * VfPq[f + i * NFN];
+  } else if (S::COLVF && idx >= S::WS_D + S::WS_W + S::WS_R && idx < S::WS) {
+    // face-point walk: step t, lane L (face point L) visits row (L mod NQ + t) mod NQ
+    const int k = idx - S::WS_D - S::WS_W - S::WS_R;
+    const int L = k % NFN;
+    const int t = (k / NFN) % NQ;
+    const int s = k / (NFN * NQ);
+    int i = L % NQ + t;
+    if (i >= NQ) i -= NQ;
+    ws[idx] = s == 0 ? 0.5 * VqLq[i + L * NQ] : -0.5 * VfPq[L + i * NFN];
   }
 }
 
@@ -393,10 +406,6 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
   // volume-face block, in H parts of NH face points.  Records and accumulator
   // clearing are spread over the whole wave (all teams' face points together).
   constexpr int TPW = S::TPW, PER = VolSmem<S>::per_elem;
-  for (int p = wc.lane; p < TPW * NFN * NFLD; p += WAVE) {
-    const int t = p / (NFN * NFLD);
-    wc.smem[t * PER + VolSmem<S>::accF_off + (p - t * NFN * NFLD)] = 0.0;
-  }
   // own-side part of the averaged normal of row r on a face: sum_q (G_qd / 2) nref_q
   auto own_normal = [&](int r, int face, double (&a)[DIM]) {
     for (int d = 0; d < DIM; ++d) {
@@ -405,6 +414,101 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
       a[d] = v;
     }
   };
+  if constexpr (S::COLVF) {
+    // Lane L takes face point L of the wave's one element and walks the rows:
+    // at step t row i = (L mod NQ + t) mod NQ.  The face point's record and
+    // its accumulator stay in registers; the volume records stay in LDS, with
+    // each row's own-side normal part for the four faces written from the half
+    // geofacs on (the face accumulator, written behind them, is stored only
+    // after the walk).  Row contributions go to the LDS accumulator, lanes
+    // L < NQ and L >= NQ in two instructions (their rows coincide), so no two
+    // lanes of an instruction add into one address and the order of the
+    // additions is fixed.  The row's register part of the volume-volume block
+    // joins its LDS part now (registers for the walk; the row sum is then the
+    // LDS value alone).  A step's loads are not issued a step ahead: that
+    // measured slower here than the compiler's schedule of the plain loop.
+    static_assert(NFLD * NQ + (VS + S::NFACES * DIM) * NQ <= PER, "normal parts inside the element's LDS");
+    if (rv[0]) {
+      for (int face = 0; face < S::NFACES; ++face) {
+        double a[DIM];
+        own_normal(0, face, a);
+        for (int d = 0; d < DIM; ++d) vrec[(VS + face * DIM + d) * NQ + l] = a[d];
+      }
+      for (int f = 0; f < NFLD; ++f) {
+        accV[f * NQ + l] = acc[0][f] + accV[f * NQ + l];
+        acc[0][f] = 0.0;
+      }
+    }
+    const int L = wc.lane;
+    const bool cv = wc.kfirst < wc.K;
+    const int face = L / NFQ;
+    Node<DIM> nf;
+    double bf[DIM];
+    if (cv) {
+      const double *Uft = wc.Uf + (size_t)wc.kfirst * NFN * NFLD;
+      const double *vfgt = wc.vfg + (size_t)wc.kfirst * NFN * NG;
+      double U[NFLD];
+      for (int f = 0; f < NFLD; ++f) U[f] = Uft[L + f * NFN];
+      nf = make_node<DIM>(U, gm1);
+      for (int d = 0; d < DIM; ++d) {
+        double b = 0.0;
+        for (int q = 0; q < DIM; ++q) b = b + 0.5 * vfgt[L + (q * DIM + d) * NFN] * s_nref[face][q];
+        bf[d] = b;
+      }
+    }
+    __syncthreads();
+    double sF[NFLD];
+    for (int f = 0; f < NFLD; ++f) sF[f] = 0.0;
+    if (cv) {
+      const double *Wc1 = ws + S::WS_D + S::WS_W + S::WS_R;
+      const double *Wc2 = Wc1 + NQ * NFN;
+      const int i0 = L < NQ ? L : L - NQ;
+      const bool second = L >= NQ;
+      struct ColStep {
+        Node<DIM> pv;
+        double a[DIM], w1, w2;
+      };
+      auto fetch = [&](int t, ColStep &x) {
+        int i = i0 + t;
+        if (i >= NQ) i -= NQ;
+        x.pv = load_node<DIM>(vrec, NQ, i);
+        for (int d = 0; d < DIM; ++d) x.a[d] = vrec[(VS + face * DIM + d) * NQ + i];
+        x.w1 = ld_off(Wc1, (unsigned)(t * NFN + L));
+        x.w2 = ld_off(Wc2, (unsigned)(t * NFN + L));
+      };
+      auto step = [&](int t, const ColStep &x) {
+        int i = i0 + t;
+        if (i >= NQ) i -= NQ;
+        double n[DIM];
+        for (int d = 0; d < DIM; ++d) n[d] = x.a[d] + bf[d];
+        const PairFlux<DIM> P = pair_flux<DIM>(x.pv, nf, inv2gm1);
+        double Fn[NFLD];
+        flux_dot<DIM>(P, n, Fn);
+        double c[NFLD];
+        for (int f = 0; f < NFLD; ++f) {
+          sF[f] += x.w2 * Fn[f];
+          c[f] = x.w1 * Fn[f];
+        }
+        for (int g = 0; g < 2; ++g)
+          if (second == (g == 1))
+            for (int f = 0; f < NFLD; ++f) lds_add(&accV[f * NQ + i], c[f]);
+      };
+      for (int t = 0; t < NQ; ++t) {
+        ColStep x;
+        fetch(t, x);
+        step(t, x);
+      }
+    }
+    __syncthreads();
+    if (cv)
+      for (int f = 0; f < NFLD; ++f) accF[f * NFN + L] = sF[f];
+    __syncthreads();
+    return;
+  }
+  for (int p = wc.lane; p < TPW * NFN * NFLD; p += WAVE) {
+    const int t = p / (NFN * NFLD);
+    wc.smem[t * PER + VolSmem<S>::accF_off + (p - t * NFN * NFLD)] = 0.0;
+  }
   for (int h = 0; h < H; ++h) {
     bool ok_f = true;
     for (int p = wc.lane; p < TPW * NH; p += WAVE) {
