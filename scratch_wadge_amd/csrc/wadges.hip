@@ -489,9 +489,10 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
           sF[f] += x.w2 * Fn[f];
           c[f] = x.w1 * Fn[f];
         }
-        for (int g = 0; g < 2; ++g)
-          if (second == (g == 1))
-            for (int f = 0; f < NFLD; ++f) lds_add(&accV[f * NQ + i], c[f]);
+        if (!second)
+          for (int f = 0; f < NFLD; ++f) lds_add(&accV[f * NQ + i], c[f]);
+        if (second)
+          for (int f = 0; f < NFLD; ++f) lds_add_second(&accV[f * NQ + i], c[f]);
       };
       for (int t = 0; t < NQ; ++t) {
         ColStep x;
