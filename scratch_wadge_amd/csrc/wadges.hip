@@ -145,11 +145,17 @@ template <class S> constexpr int vf_steps() { return face_chunked<S>() ? S::NFN 
 // tetrahedra whose lanes form at most two groups visit the faces in two halves
 // (faces 0-1, then 2-3), the two groups on different faces of the half at every
 // step (the steps t and their partners are those of vf_partner, taken in another
-// order).  With R > 1 rows per lane an element's lanes form one group, which
-// walks the faces one at a time.
+// order).  Face-chunked triangles whose lanes form two groups (N = 1, 2) walk
+// the faces one at a time, both groups on the same face (their face-accumulator
+// additions then go in two instructions, one per group): the smaller LDS
+// footprint matters to the ensemble kernel, whose LDS per wave is that of its
+// largest shape.  With R > 1 rows per lane an element's lanes form one group,
+// which walks the faces one at a time.
+template <class S> constexpr int face_groups() { return (S::NQ + S::NFQ - 1) / S::NFQ; }
 template <class S> constexpr int face_parts() {
   if constexpr (S::R > 1) return S::NFACES;
-  return (face_chunked<S>() && S::NFACES == 4 && (S::NQ + S::NFQ - 1) / S::NFQ <= 2) ? 2 : 1;
+  if constexpr (face_chunked<S>() && S::DIM == 2 && face_groups<S>() == 2) return S::NFACES;
+  return (face_chunked<S>() && S::NFACES == 4 && face_groups<S>() <= 2) ? 2 : 1;
 }
 
 // ------------------------------------------------------------------ operator prep
@@ -610,7 +616,11 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
         const double *W1 = ws + S::WS_D;
         const double *W2 = W1 + T * NQ;
         // one volume-face pair: step t, partner face point fn (record fl), own-side normal part a
-        auto vf_pair = [&](int t, int fn, int fl, const double (&a)[DIM]) {
+        // groups of lanes that meet on one face point in a step (faces per part
+        // fewer than groups) add into the face accumulator in two instructions
+        constexpr bool SPLIT = face_chunked<S>() && face_groups<S>() > S::NFACES / H;
+        static_assert(!SPLIT || face_groups<S>() == 2, "two lane groups at most per face");
+        auto vf_pair = [&](int t, int fn, int fl, const double (&a)[DIM], bool second) {
           const Node<DIM> pf = load_node<DIM>(frec, NH, fl);
           double n[DIM];
           for (int d = 0; d < DIM; ++d) n[d] = a[d] + frec[(VS + d) * NH + fl];
@@ -619,10 +629,11 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
           flux_dot<DIM>(P, n, Fn);
           const double w1 = ld_off(W1, (unsigned)(t * NQ + i));
           const double w2 = ld_off(W2, (unsigned)(t * NQ + i));
-          for (int f = 0; f < NFLD; ++f) {
-            acc[0][f] += w1 * Fn[f];
-            lds_add(&accF[f * NFN + fn], w2 * Fn[f]);
-          }
+          for (int f = 0; f < NFLD; ++f) acc[0][f] += w1 * Fn[f];
+          if (!SPLIT || !second)
+            for (int f = 0; f < NFLD; ++f) lds_add(&accF[f * NFN + fn], w2 * Fn[f]);
+          if (SPLIT && second)
+            for (int f = 0; f < NFLD; ++f) lds_add_second(&accF[f * NFN + fn], w2 * Fn[f]);
         };
         if constexpr (face_chunked<S>()) {
           const int g = i / NFQ, o = i - (i / NFQ) * NFQ;
@@ -646,7 +657,7 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
                 int node = o + m;
                 if (node >= NFQ) node -= NFQ;
                 const int fn = face * NFQ + node;
-                vf_pair(c * NFQ + m, fn, fn - h * NH, a);
+                vf_pair(c * NFQ + m, fn, fn - h * NH, a, g > 0);
               }
             }
           }
@@ -656,7 +667,7 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
             if (fn >= NFN) continue;
             double a[DIM];
             own_normal(0, fn / NFQ, a);
-            vf_pair(t, fn, fn, a);
+            vf_pair(t, fn, fn, a, false);
           }
         }
       }

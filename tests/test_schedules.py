@@ -31,12 +31,16 @@ def shape(dim, nq, nfq):
     tpw = WAVE // lpe
     colvf = r == 1 and tpw == 1 and nfn == WAVE and nq < WAVE
     chunked = nq <= nfn
+    groups = -(-nq // nfq)
     if r > 1:
         parts = nfaces
+    elif chunked and dim == 2 and groups == 2:
+        parts = nfaces
     else:
-        parts = 2 if (chunked and nfaces == 4 and -(-nq // nfq) <= 2) else 1
+        parts = 2 if (chunked and nfaces == 4 and groups <= 2) else 1
+    split = chunked and groups > nfaces // parts
     return dict(dim=dim, nq=nq, nfq=nfq, nfaces=nfaces, nfn=nfn, r=r, lpe=lpe, colvf=colvf,
-                chunked=chunked, parts=parts)
+                chunked=chunked, parts=parts, split=split)
 
 
 def vv_schedule(s):
@@ -84,7 +88,8 @@ def vf_schedule(s):
                         g, o = divmod(i, nfq)
                         face = h * fh + (u + g) % fh
                         fn = face * nfq + (o + m) % nfq
-                        yield (h, u, m), i, fn, ("F", fn)
+                        # groups meeting on a face add in one instruction each
+                        yield (h, u, m, g if s["split"] else 0), i, fn, ("F", fn)
     else:
         L = max(nq, nfn)
         for t in range(L):
@@ -123,6 +128,19 @@ def test_volume_face_pairs_once_and_adds_distinct(dim, nq, nfq):
     assert all(c == 1 for c in seen.values())
     for key, addrs in per_instr.items():
         assert len(addrs) == len(set(addrs)), (key, addrs)
+
+
+def test_triangle_face_walk_splits_its_two_groups():
+    """Triangles with two lane groups walk one face at a time: both groups meet
+    on the face's points, so one instruction for both would add twice into an
+    address; the kernel adds per group."""
+    for nq, nfq in ((3, 2), (6, 3)):
+        s = shape(2, nq, nfq)
+        assert s["parts"] == 3 and s["split"]
+        per_step = {}
+        for key, i, fn, addr in vf_schedule(s):
+            per_step.setdefault(key[:3], []).append(addr)
+        assert any(len(a) != len(set(a)) for a in per_step.values())
 
 
 def test_one_element_walk_needs_its_two_lane_groups():
