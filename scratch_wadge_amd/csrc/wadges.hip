@@ -1105,10 +1105,12 @@ __device__ __forceinline__ void stage_body(const ConstArgs *A, double fa, double
   vol_body<S>(valid, l, A->rhsQ_in + base, A->vgeo + kk * NQ * NG, s_nref, A->ws, accV, gm1,
               inv2gm1, acc, wc);
 
-  // The update's per-row inputs.  With R > 1 they are loaded now, so that their
-  // HBM latency passes under the surface and lift phases (the other shapes
-  // have no registers to spare for them and load them where they are used).
-  constexpr bool EARLY = R > 1;
+  // The update's per-row inputs.  With R > 1 and in 2D they are loaded now, so
+  // that their HBM latency passes under the surface and lift phases (2D: the
+  // ensemble 3 % and the single-problem stage 4 % faster).  The 3D one-row
+  // shapes load them where they are used: N = 2 has no registers to spare, and
+  // at N = 4 loading them early measured the same.
+  constexpr bool EARLY = R > 1 || DIM == 2;
   int ir[R];
   for (int r = 0; r < R; ++r) ir[r] = rv[r] ? l + r * LPE : 0;
   double Qin[EARLY ? R : 1][NFLD], resin[EARLY ? R : 1][NFLD], J[R];
