@@ -342,7 +342,7 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
     // With R > 1 the operator entries of step t + 1 are loaded while step t
     // computes (one row per lane has no registers to spare for it).  (Two
     // buffers used in turn, as in the face walk below, spilled here.)
-    constexpr bool PIPE = R > 1;
+    constexpr bool PIPE = R > 1 || DIM == 2;
     const double *Dfw = ws;
     const double *Dbw = ws + DIM * TVT * NQ;
     int iw[R];                        // table row (row 0's for rows that do not exist)
@@ -620,15 +620,14 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
         // fewer than groups) add into the face accumulator in two instructions
         constexpr bool SPLIT = face_chunked<S>() && face_groups<S>() > S::NFACES / H;
         static_assert(!SPLIT || face_groups<S>() == 2, "two lane groups at most per face");
-        auto vf_pair = [&](int t, int fn, int fl, const double (&a)[DIM], bool second) {
+        auto vf_pair = [&](int fn, int fl, const double (&a)[DIM], bool second, double w1,
+                           double w2) {
           const Node<DIM> pf = load_node<DIM>(frec, NH, fl);
           double n[DIM];
           for (int d = 0; d < DIM; ++d) n[d] = a[d] + frec[(VS + d) * NH + fl];
           const PairFlux<DIM> P = pair_flux<DIM>(me[0], pf, inv2gm1, smooth_f);
           double Fn[NFLD];
           flux_dot<DIM>(P, n, Fn);
-          const double w1 = ld_off(W1, (unsigned)(t * NQ + i));
-          const double w2 = ld_off(W2, (unsigned)(t * NQ + i));
           for (int f = 0; f < NFLD; ++f) acc[0][f] += w1 * Fn[f];
           if (!SPLIT || !second)
             for (int f = 0; f < NFLD; ++f) lds_add(&accF[f * NFN + fn], w2 * Fn[f]);
@@ -653,11 +652,31 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
               if (c < 0) c += S::NFACES;
               double a[DIM];
               own_normal(0, face, a);
+              // 2D: a step's operator entries are loaded during the step before
+              constexpr bool VFPIPE = DIM == 2;
+              double w1n = 0.0, w2n = 0.0;
+              if (VFPIPE) {
+                w1n = ld_off(W1, (unsigned)(c * NFQ * NQ + i));
+                w2n = ld_off(W2, (unsigned)(c * NFQ * NQ + i));
+              }
               for (int m = 0; m < m_steps; ++m) {
                 int node = o + m;
                 if (node >= NFQ) node -= NFQ;
                 const int fn = face * NFQ + node;
-                vf_pair(c * NFQ + m, fn, fn - h * NH, a, g > 0);
+                const int t = c * NFQ + m;
+                double w1, w2;
+                if (VFPIPE) {
+                  w1 = w1n;
+                  w2 = w2n;
+                  if (m + 1 < m_steps) {
+                    w1n = ld_off(W1, (unsigned)((t + 1) * NQ + i));
+                    w2n = ld_off(W2, (unsigned)((t + 1) * NQ + i));
+                  }
+                } else {
+                  w1 = ld_off(W1, (unsigned)(t * NQ + i));
+                  w2 = ld_off(W2, (unsigned)(t * NQ + i));
+                }
+                vf_pair(fn, fn - h * NH, a, g > 0, w1, w2);
               }
             }
           }
@@ -667,7 +686,8 @@ __device__ __forceinline__ void vol_body(bool valid, int l, const double *Uq, co
             if (fn >= NFN) continue;
             double a[DIM];
             own_normal(0, fn / NFQ, a);
-            vf_pair(t, fn, fn, a, false);
+            vf_pair(fn, fn, a, false, ld_off(W1, (unsigned)(t * NQ + i)),
+                    ld_off(W2, (unsigned)(t * NQ + i)));
           }
         }
       }
