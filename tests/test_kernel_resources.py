@@ -2,10 +2,9 @@
 (-Rpass-analysis=kernel-resource-usage) on a device-only compile with the
 library's own flags.  The volume and fused stage kernels of every shape must not
 spill to scratch (a spill there costs more than any register-hungry schedule
-gains); the one-tile ensemble kernels may spill at most 32 bytes per lane (the
-2D one trades 20 bytes for a fourth wave per SIMD, measured 6 % faster on the
-64-problem ensemble); the 3D N = 3 fused stage must keep the two waves per SIMD
-that its LDS allows."""
+gains); the one-tile ensemble kernels may spill at most 32 bytes per lane (a
+fourth wave per SIMD was once worth a 20-byte spill in the 2D one); the 3D N = 3
+fused stage must keep the two waves per SIMD that its LDS allows."""
 
 import os
 import re
