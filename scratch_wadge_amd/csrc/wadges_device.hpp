@@ -260,9 +260,10 @@ __device__ __forceinline__ void lds_add(double *p, double v) {
 
 // lds_add for a second group of lanes whose addresses coincide with the first
 // group's (taken under the complementary lane mask right after it).  Its
-// atomic carries another scope (the wave: a workgroup here is one wave), so the
-// compiler cannot merge the two masked adds into one instruction in which two
-// lanes would add into one address in an order the program does not fix.
+// atomic carries another scope, the wave's (the addresses lie in the calling
+// wave's own LDS region, which no other wave touches), so the compiler cannot
+// merge the two masked adds into one instruction in which two lanes would add
+// into one address in an order the program does not fix.
 __device__ __forceinline__ void lds_add_second(double *p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
