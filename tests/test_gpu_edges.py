@@ -15,7 +15,7 @@ from scratch_wadge_amd.problem import RK4A, RK4B
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(2, 3), (3, 3), (3, 4)]
+CASES = [(2, 1), (2, 2), (2, 3), (3, 3), (3, 4)]
 
 
 def _geo(dev):
